@@ -1,0 +1,268 @@
+"""Throughput benchmark of the VMC sampling hot path on one MI355X.
+
+Workload (the headline configuration): 2-D quantum dot, N=56 electrons,
+omega=0.1, Coulomb, Slater(Hermite, 28x28 per spin) x Gaussian x Pade-Jastrow
+(alpha = gamma = 1, the reference's Automatize -> Constant(1.0) start),
+Langevin importance sampling with the exact Metropolis-Hastings ratio,
+dt=0.005, 32768 walkers.  A "step" of this benchmark is one sampling iteration:
+every walker runs `--steps-per-iter` single-coordinate Metropolis steps, each
+followed by the full local energy and dlnpsi/dtheta sample (the reference's
+per-step semantics, ref src/system.cpp:147-164, src/sampler.cpp:52-90).
+
+value = walker-steps/s of the FP32 walk (FP64 Slater-inverse rebuild every
+`--refresh` steps, FP64 accumulation).  The plain FP64 walk, the integrated
+autocorrelation time of E_L and the CPU baseline are reported beside it.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic cost of one walker-step at N=56 (single-coordinate move,
+# acceptance a ~ 1, n = 28 per spin block, D = 2):
+#   flops  F = 2n + a*4n^2 + 4D n^2 + 16 n(1+2D) + 40(N-1) + 6ND
+#   bytes  B = b [(1+a) n^2 + 2D n^2 + (2D+1) n + 3ND]   (walker state a streaming
+#              implementation would touch; b = 4 (FP32) or 8 (FP64))
+def flops_per_step(N, n, D=2, a=1.0):
+    return 2 * n + a * 4 * n * n + 4 * D * n * n + 16 * n * (1 + 2 * D) + 40 * (N - 1) + 6 * N * D
+
+
+def bytes_per_step(N, n, b, D=2, a=1.0):
+    return b * ((1 + a) * n * n + 2 * D * n * n + (2 * D + 1) * n + 3 * N * D)
+
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def make_engine(V, args, precision, walkers):
+    cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
+                        sampling=V.IMPORTANCE_EXACT, step_length=args.dt, quirks=0,
+                        drift_limit=args.drift_limit,
+                        precision=precision,
+                        refresh_interval=args.refresh if precision == 32 else 0,
+                        num_walkers=walkers)
+    eng = V.Engine(cfg)
+    eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
+    eng.init_walkers(args.seed)
+    return eng
+
+
+def burn_in(eng, steps, label):
+    done = 0
+    t0 = time.time()
+    while done < steps:
+        chunk = min(steps - done, 20000)
+        s, _, _, _ = eng.run(chunk, 0)
+        done += chunk
+        print(f"# {label}: burn-in {done}/{steps} steps/walker, <E> = "
+              f"{s['energy'] / s['samples']:.4f}, acc = {s['accepted'] / s['samples']:.4f} "
+              f"({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+
+def timed(eng, iters, warmup, spi):
+    for _ in range(warmup):
+        eng.run(spi, 0)
+    kern = 0.0
+    E = E2 = n = acc = 0.0
+    t0 = time.perf_counter()
+    for it in range(iters):
+        s, _, _, _ = eng.run(spi, 0, it, 1)
+        kern += s["kernel_ms"]
+        E += s["energy"]
+        E2 += s["energy2"]
+        n += s["samples"]
+        acc += s["accepted"]
+    wall = time.perf_counter() - t0
+    return wall, kern / 1000.0, E / n, acc / n
+
+
+def measure_iat(eng, steps, rec_walkers):
+    from vmachine_amd import stats
+    _, _, _, en = eng.run(steps, 0, record_walkers=rec_walkers)
+    en = en.astype(np.float64)
+    tau_b = stats.iat_many(en, "blocking")
+    tau_s = stats.iat_many(en[: min(64, rec_walkers)], "sokal")
+    means = en.mean(axis=1)
+    return tau_b, tau_s, float(means.mean()), float(means.std() / np.sqrt(len(means)))
+
+
+def cpu_baseline(args):
+    """The reference binary (oracle/_ref/vmachine, built from the reference's own
+    sources) on the host cores, one MPI rank per core, on a bounded sample of the
+    same workload; falls back to the oracle's multithreaded restatement."""
+    cores = min(16, os.cpu_count() or 1)
+    ref = os.path.join(ROOT, "oracle", "_ref", "vmachine")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if os.path.exists(ref) and os.path.exists(mpiexec) and not args.cpu_port:
+        per_rank = args.cpu_steps_per_rank
+        text = (f"numParticles: {args.N}\nnumDimensions: 2\nhamiltonian: harmonicOscillator\n"
+                f"omega: {args.omega}\ninteractionStyle: coulomb\n"
+                "waveFunctionElement: slaterDeterminant\nwaveFunctionElement: gaussian\n"
+                "waveFunctionElement: padeJastrow\nbasis: hermite\n"
+                f"numIterations: 2\nnumSteps: {per_rank * cores}\nlearningRate: 0.0\n"
+                f"stepLength: {args.dt}\nsampling: importanceSampling\n")
+        with tempfile.TemporaryDirectory() as td:
+            with open(os.path.join(td, "input.in"), "w") as f:
+                f.write(text)
+            try:
+                t0 = time.time()
+                r = subprocess.run([mpiexec, "-n", str(cores), ref, "input.in"], cwd=td,
+                                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                                   timeout=600)
+                wall = time.time() - t0
+                rows = [ln.split() for ln in r.stdout.splitlines()
+                        if ln[:1].isdigit() and len(ln.split()) == 11]
+                if r.returncode == 0 and rows:
+                    # last column: sampling time summed over ranks for one iteration
+                    t_sum = float(rows[0][-1])
+                    steps = per_rank * cores + int(per_rank * cores * 0.001) * cores
+                    rate = steps / (t_sum / cores)
+                    return {"value": rate, "unit": "walker-steps/s", "cores": cores,
+                            "kind": "reference",
+                            "sample": f"reference vmachine (FP64, importance sampling), "
+                                      f"mpiexec -n {cores}, {per_rank} steps/rank, one "
+                                      f"iteration timed by its own MPI_Wtime (wall {wall:.1f} s "
+                                      f"incl. start-up)"}
+            except (OSError, subprocess.SubprocessError) as ex:
+                print(f"# reference baseline failed: {ex}", file=sys.stderr)
+    from oracle import oracle as O
+    cfg = O.make_config(args.N, 2, args.omega, 1, [O.SLATER, O.GAUSSIAN, O.PADE], None, None,
+                        False, O.IMPORTANCE_EXACT, args.dt, quirks=0)
+    W = cores * 4
+    pos = np.stack([O.initial_positions(args.seed, w, 2 * args.N, args.omega) for w in range(W)])
+    P = np.array([[1.0], [1.0], [1.0]])
+    counters = np.zeros(W, dtype=np.uint64)
+    steps = args.cpu_steps_per_rank // 4
+    t0 = time.perf_counter()
+    O.run_many(cfg, P, pos, args.seed, counters, steps, 0, threads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": W * steps / dt, "unit": "walker-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle restatement (FP64), {W} walkers x {steps} steps on {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed sampling iterations")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps-per-iter", type=int, default=1000)
+    ap.add_argument("--walkers", type=int, default=32768)
+    ap.add_argument("--N", type=int, default=56)
+    ap.add_argument("--omega", type=float, default=0.1)
+    ap.add_argument("--dt", type=float, default=0.005)
+    ap.add_argument("--refresh", type=int, default=100)
+    ap.add_argument("--drift-limit", type=float, default=1.0,
+                    help="Umrigar drift limit a of the exact Langevin proposal (0: none)")
+    ap.add_argument("--burnin", type=int, default=40000, help="burn-in steps per walker")
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--iat-steps", type=int, default=8192)
+    ap.add_argument("--iat-walkers", type=int, default=512)
+    ap.add_argument("--fp64-iters", type=int, default=3)
+    ap.add_argument("--cpu-steps-per-rank", type=int, default=200000)
+    ap.add_argument("--cpu-port", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-iat", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this engine runs independent walkers on one device "
+              "and no multi-GPU figure is produced (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import vmachine_amd as V
+    if V.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    n = args.N // 2 if args.N > 2 else 1
+    n = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28}[args.N]
+
+    # ---- FP32 walk (value) ----
+    eng = make_engine(V, args, 32, args.walkers)
+    burn_in(eng, args.burnin, "fp32")
+    wall, kern, E32, acc32 = timed(eng, args.steps, args.warmup, args.steps_per_iter)
+    wsteps = args.walkers * args.steps_per_iter * args.steps
+    value = wsteps / wall
+    kern_rate = wsteps / kern
+    launches = args.steps * ((args.steps_per_iter + args.refresh - 1) // args.refresh)
+    launch_s = kern / launches
+    steps_per_launch = args.walkers * min(args.refresh, args.steps_per_iter)
+    F = flops_per_step(args.N, n, a=acc32)
+    B4 = bytes_per_step(args.N, n, 4, a=acc32)
+    achieved = F * steps_per_launch / launch_s / 1e12
+    iat = None
+    if not args.no_iat:
+        tb, ts, Em, Ee = measure_iat(eng, args.iat_steps, args.iat_walkers)
+        iat = {"tau_int_blocking": tb, "tau_int_sokal": ts, "energy": Em, "energy_err": Ee}
+    eng.close()
+
+    # ---- plain FP64 walk ----
+    fp64 = None
+    if not args.no_fp64:
+        eng = make_engine(V, args, 64, args.walkers)
+        burn_in(eng, args.burnin, "fp64")
+        w64, k64, E64, a64 = timed(eng, args.fp64_iters, 1, args.steps_per_iter)
+        r64 = args.walkers * args.steps_per_iter * args.fp64_iters / w64
+        fp64 = {"walker_steps_per_s": r64, "energy": E64, "acceptance": a64,
+                "kernel_walker_steps_per_s": args.walkers * args.steps_per_iter *
+                args.fp64_iters / k64}
+        if not args.no_iat:
+            tb, ts, Em, Ee = measure_iat(eng, args.iat_steps, args.iat_walkers)
+            fp64.update({"tau_int_blocking": tb, "energy_iat_run": Em, "energy_err": Ee,
+                         "iat_corrected_samples_per_s": r64 / (2 * tb)})
+        eng.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(args)
+
+    out = {
+        "metric": "walker-steps/s and IAT-corrected samples/s at N=56 w=0.1, vs plain FP64",
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 walk, f64 inverse rebuild and accumulation",
+        "data": "synthetic (walkers drawn from N(0, 1/omega^2) per coordinate, burned in)",
+        "config": {"workload": f"2-D quantum dot N={args.N} omega={args.omega}, Coulomb, "
+                               "Slater(Hermite) x Gaussian x Pade-Jastrow, exact MH Langevin "
+                               f"dt={args.dt}, drift limit a={args.drift_limit}",
+                   "walkers": args.walkers, "steps_per_iter": args.steps_per_iter,
+                   "refresh_interval": args.refresh, "burnin_steps": args.burnin},
+        "kernel_walker_steps_per_s": kern_rate,
+        "energy": E32,
+        "acceptance": acc32,
+        "iat": iat,
+        "iat_corrected_samples_per_s": (value / (2 * iat["tau_int_blocking"])) if iat else None,
+        "fp64": fp64,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "note": "VALU FP32 compute roofline (no MFMA in the walk kernel; walker "
+                             "state is register-resident); algorithmic flops "
+                             f"{F:.0f}/walker-step x {steps_per_launch} walker-steps per launch; "
+                             f"streaming-equivalent bytes {B4:.0f}/walker-step"},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

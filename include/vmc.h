@@ -1,0 +1,174 @@
+/*
+ * vmc.h -- C ABI of the MI355X VMC sampling engine (libvmc_hip.so).
+ *
+ * The reference (evenmn/VMaChine) has no foreign-function interface: its hot
+ * path is a set of C++ plugin classes called once per Metropolis step from
+ * System::runMetropolisCycles.  A per-step, per-element virtual call cannot
+ * cross a device boundary, so this ABI sits one level up, at the granularity
+ * of "advance W walkers through one sampling iteration and return the
+ * Sampler's sums".  Each entry point names the reference interface it
+ * replaces.  Plain C types only; every call returns 0 on success or a negative
+ * VMC_E* code, with a message from vmc_last_error().
+ */
+#ifndef VMC_H
+#define VMC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define VMC_OK 0
+#define VMC_E_INVALID (-1)   /* bad argument or unsupported configuration */
+#define VMC_E_DEVICE (-2)    /* HIP runtime error / no device */
+#define VMC_E_STATE (-3)     /* call out of order (e.g. run before parameters) */
+
+/* ---- wave-function elements (ref src/parser.cpp:263-286 keys) ---- */
+#define VMC_GAUSSIAN 1       /* waveFunctionElement: gaussian          (ref WaveFunctions/gaussian.cpp) */
+#define VMC_SLATER 2         /* waveFunctionElement: slaterDeterminant (ref WaveFunctions/slaterdeterminant.cpp) */
+#define VMC_PADE 3           /* waveFunctionElement: padeJastrow       (ref WaveFunctions/padejastrow.cpp) */
+#define VMC_RBM_GAUSSIAN 4   /* waveFunctionElement: RBMGaussian       (ref WaveFunctions/rbmgaussian.cpp) */
+#define VMC_RBM_PRODUCT 5    /* waveFunctionElement: RBMProduct        (ref WaveFunctions/rbmproduct.cpp) */
+
+/* ---- sampling (ref src/parser.cpp:210-222) ---- */
+#define VMC_BRUTE_FORCE 0         /* sampling: bruteForce (ref Metropolis/bruteforce.cpp:26-52) */
+#define VMC_IMPORTANCE 1          /* sampling: importanceSampling, the reference's Green's-function ratio
+                                     (ref Metropolis/importancesampling.cpp:43-47, 68-81) */
+#define VMC_IMPORTANCE_EXACT 2    /* Langevin proposal with the exact Metropolis-Hastings ratio */
+
+/* ---- reference behaviours that change results (set bit = reproduce) ---- */
+#define VMC_Q2_PADE_RATIO_UPPER (1u << 2)  /* ref padejastrow.cpp:214 */
+#define VMC_Q3_PADE_GAMMA_HALF (1u << 3)   /* ref padejastrow.cpp:114 */
+#define VMC_QUIRKS_REFERENCE (VMC_Q2_PADE_RATIO_UPPER | VMC_Q3_PADE_GAMMA_HALF)
+
+typedef struct vmc_config {
+    int num_particles;      /* numParticles */
+    int num_dimensions;     /* numDimensions (the engine supports 2) */
+    double omega;           /* omega */
+    int interaction;        /* interactionStyle: 0 noInteraction, 1 coulomb */
+    int num_elements;       /* waveFunctionElement lines, in order: row e of the parameter matrix */
+    int kinds[6];
+    int num_hidden;         /* numHiddenNodes */
+    double sigma;           /* RBM width (1/sqrt(omega) once omega is set) */
+    int sorting;            /* sorting */
+    int sampling;           /* VMC_BRUTE_FORCE / VMC_IMPORTANCE / VMC_IMPORTANCE_EXACT */
+    double step_length;     /* stepLength */
+    unsigned quirks;        /* VMC_Q* bits */
+    int precision;          /* 64: FP64 walk; 32: FP32 walk with FP64 inverse rebuilds */
+    int refresh_interval;   /* steps between FP64 Slater-inverse rebuilds (0: launch start only) */
+    int num_walkers;        /* independent chains on the device */
+    double drift_limit;     /* VMC_IMPORTANCE_EXACT only: a > 0 replaces the drift F by
+                               2F / (1 + sqrt(1 + a F^2 dt / 2)) in the proposal and in its
+                               density (Umrigar, Nightingale, Runge, J. Chem. Phys. 99, 2865
+                               (1993)), so walkers next to a node of the Slater determinant
+                               are not frozen by its divergent drift; 0 = textbook Langevin */
+} vmc_config;
+
+/* Sums over all walkers of one sampling iteration (ref src/sampler.cpp:52-90,
+ * the quantities Sampler::computeTotals reduces, ref src/sampler.cpp:111-187). */
+typedef struct vmc_sums {
+    double energy, energy2;
+    double kinetic, external, interaction;
+    double kinetic2, external2, interaction2;
+    double accepted;          /* accepted moves among the sampled steps */
+    double samples;           /* walkers x sampled steps */
+    double gradient_samples;  /* walkers x gradient-window length (divisor of <G>, <GE>) */
+    double kernel_ms;         /* walk-kernel time, HIP events on the engine stream */
+    double nonfinite;         /* sampled non-finite local energies (all walkers) */
+} vmc_sums;
+
+typedef struct vmc_engine vmc_engine;
+
+/* Last error message of the calling thread ("" if none). */
+const char* vmc_last_error(void);
+
+/* Parameter-matrix shape: ref WaveFunction::getNumberOfParameters and
+ * System::setMaxParameters (ref src/elementoperation.cpp:173-186). Host only. */
+int vmc_num_parameters(const vmc_config* cfg, int element);
+int vmc_max_parameters(const vmc_config* cfg);
+
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+int vmc_device_count(void);
+
+/* Creates the engine on the current HIP device.  Replaces the construction
+ * and System::initializeSystem wiring of the plugins (ref src/system.cpp:50-73). */
+int vmc_engine_create(const vmc_config* cfg, vmc_engine** out);
+void vmc_engine_destroy(vmc_engine* e);
+
+/* Parameter matrix, row-major [rows = num_elements][cols = max parameters].
+ * Replaces System::updateAllParameters (ref src/elementoperation.cpp:68-73). */
+int vmc_set_parameters(vmc_engine* e, const double* params, int rows, int cols);
+
+/* Draws every walker's positions x ~ N(0, (1/omega)^2) from Philox(seed) and
+ * resets the step counter.  Replaces RandomNormal::setupInitialState
+ * (ref src/InitialStates/randomnormal.cpp:10-25) and
+ * ImportanceSampling::initialize (ref src/Metropolis/importancesampling.cpp:11-32). */
+int vmc_init_walkers(vmc_engine* e, uint64_t seed);
+
+/* Walker positions, row-major [walkers][num_particles*num_dimensions], FP64. */
+int vmc_set_positions(vmc_engine* e, const double* positions, int walkers, int dof);
+int vmc_get_positions(vmc_engine* e, double* positions, int walkers, int dof);
+/* The importance sampler's stored drift vector per walker (same layout). */
+int vmc_set_quantum_force(vmc_engine* e, const double* qf, int walkers, int dof);
+int vmc_get_quantum_force(vmc_engine* e, double* qf, int walkers, int dof);
+/* Random-number state: the Philox key (seed) and the step counter of the next
+ * step (all walkers share both; walker w draws from counter words (step, w)).
+ * Replaces RandomNumberGenerator::setSeed (ref src/RNG/rng.h:10). */
+int vmc_set_random_state(vmc_engine* e, uint64_t seed, uint64_t counter);
+int vmc_get_random_state(vmc_engine* e, uint64_t* seed, uint64_t* counter);
+
+/* One sampling iteration: every walker runs `equilibration` burn-in steps and
+ * `steps` sampled steps; the gradient sums cover the steps t with
+ * S*(iteration % batches) < t < S*(iteration % batches + 1), S = steps/batches,
+ * t counted from the first burn-in step.  grad and grad_e (row-major,
+ * num_elements x max parameters) receive the window sums of dlnpsi/dtheta and
+ * E_L*dlnpsi/dtheta.  Replaces System::runMetropolisCycles +
+ * Sampler::sample/computeTotals (ref src/system.cpp:147-164,
+ * src/sampler.cpp:52-90, 111-187). */
+int vmc_run(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+            vmc_sums* sums, double* grad, double* grad_e);
+
+/* As vmc_run, and also returns the sampled local energies of the first
+ * `record_walkers` walkers, row-major [record_walkers][steps] (FP32).
+ * Replaces Sampler::printInstantValuesToFile (ref src/sampler.cpp:624-632). */
+int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+                     vmc_sums* sums, double* grad, double* grad_e, int record_walkers,
+                     float* energies);
+
+/* Parity entry point.  For each of `walkers` configurations (positions
+ * row-major [walkers][dof]) the engine rebuilds its state, evaluates the local
+ * energy and its parts, dlnpsi/dtheta and the drift, then forces the move of
+ * coordinate moves[w] to new_values[w] through its incremental update path and
+ * evaluates again.  out_before / out_after: [walkers][VMC_EVAL_WIDTH] FP64.
+ * Row layout: see VMC_EVAL_*.  Replaces the per-element calls
+ * evaluateRatio / computeGradient / computeLaplacian / computeParameterGradient
+ * and System::getKineticEnergy (ref src/WaveFunctions/wavefunction.h:29-32,
+ * src/elementoperation.cpp:80-123). */
+#define VMC_EVAL_E 0
+#define VMC_EVAL_T 1
+#define VMC_EVAL_VEXT 2
+#define VMC_EVAL_VINT 3
+#define VMC_EVAL_GALPHA 4
+#define VMC_EVAL_GGAMMA 5
+#define VMC_EVAL_RATIO_SLATER 6   /* |det'/det|^2 of the forced move (after only) */
+#define VMC_EVAL_RATIO_PADE 7     /* Pade ratio of the forced move (after only) */
+#define VMC_EVAL_RATIO_GAUSS 8    /* Gaussian ratio of the forced move (after only) */
+#define VMC_EVAL_DRIFT 9          /* then dof drift components 2 dlnpsi/dx_k */
+#define VMC_EVAL_WIDTH(dof) (9 + (dof))
+int vmc_evaluate(vmc_engine* e, const double* positions, int walkers, const int* moves,
+                 const double* new_values, double* out_before, double* out_after);
+
+/* Parity entry point for the random-number protocol: the draws of steps
+ * counter0 .. counter0+n-1 of one walker exactly as the walk computes them in
+ * the given precision (32 or 64): coordinate index in [0, dof), standard
+ * normal, move uniform and accept uniform.  Replaces the calls
+ * RandomNumberGenerator::nextInt/nextGaussian/nextDouble of one Metropolis step
+ * (ref src/Metropolis/importancesampling.cpp:51-75, src/RNG/rng.h:11-13). */
+int vmc_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int n, int dof,
+              int* coords, double* normals, double* u_move, double* u_accept);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

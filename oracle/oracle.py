@@ -32,7 +32,7 @@ class Config(ctypes.Structure):
         ("N", _I), ("D", _I), ("omega", _D), ("interaction", _I),
         ("num_elements", _I), ("kinds", _I * 6), ("num_hidden", _I),
         ("sigma", _D), ("sorting", _I), ("sampling", _I), ("step_length", _D),
-        ("quirks", ctypes.c_uint), ("basis_scaled", _I),
+        ("quirks", ctypes.c_uint), ("basis_scaled", _I), ("drift_limit", _D),
     ]
 
 
@@ -107,8 +107,9 @@ def _dp(a):
 
 def make_config(N, D=2, omega=1.0, interaction=1, kinds=(SLATER, GAUSSIAN, PADE),
                 num_hidden=None, sigma=None, sorting=False, sampling=IMPORTANCE_COMPAT,
-                step_length=0.01, quirks=QUIRKS_ALL, basis_scaled=False):
+                step_length=0.01, quirks=QUIRKS_ALL, basis_scaled=False, drift_limit=0.0):
     c = Config()
+    c.drift_limit = drift_limit
     c.N, c.D, c.omega, c.interaction = N, D, omega, int(interaction)
     c.num_elements = len(kinds)
     for i, k in enumerate(kinds):

@@ -53,8 +53,9 @@ void vmco_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t o
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* 24-bit uniform in (0,1), exactly representable in float and double */
-static double u24(uint32_t u) { return ((double)(u >> 8) + 0.5) * (1.0 / 16777216.0); }
+/* 23-bit uniform in (0,1): (m + 1/2) 2^-23 with m < 2^23 needs 24 significant
+ * bits, so it is exact in float and double alike */
+static double u23(uint32_t u) { return ((double)(u >> 9) + 0.5) * (1.0 / 8388608.0); }
 
 static const double TWO_PI = 6.283185307179586476925286766559;
 
@@ -66,10 +67,10 @@ void vmco_step_draws(uint64_t seed, uint32_t walker, uint64_t counter, int nd,
     uint32_t o[4];
     vmco_philox4x32_10(ctr, key, o);
     *k = (int)(((uint64_t)o[0] * (uint64_t)nd) >> 32);
-    double U1 = u24(o[1]), U2 = u24(o[2]);
+    double U1 = u23(o[1]), U2 = u23(o[2]);
     *z = sqrt(-2.0 * log(U1)) * cos(TWO_PI * U2);
     *u_move = U1;
-    *u_acc = u24(o[3]);
+    *u_acc = u23(o[3]);
 }
 
 void vmco_initial_positions(uint64_t seed, uint32_t walker, int nd, double omega, double* out)
@@ -79,8 +80,8 @@ void vmco_initial_positions(uint64_t seed, uint32_t walker, int nd, double omega
         uint32_t ctr[4] = {(uint32_t)blk, 0u, walker, 1u};
         uint32_t o[4];
         vmco_philox4x32_10(ctr, key, o);
-        double r = sqrt(-2.0 * log(u24(o[0])));
-        double th = TWO_PI * u24(o[1]);
+        double r = sqrt(-2.0 * log(u23(o[0])));
+        double th = TWO_PI * u23(o[1]);
         /* std::normal_distribution(mean, 1/omega) takes a standard deviation
          * (ref InitialStates/randomnormal.cpp:17, RNG/mersennetwister.cpp:67) */
         out[2 * blk] = r * cos(th) / omega;
@@ -875,6 +876,14 @@ void vmco_reset_quantum_force(vmco_walker* w)          /* importancesampling.cpp
 void vmco_get_quantum_force(const vmco_walker* w, double* out) { memcpy(out, w->qf, sizeof(double) * w->ND); }
 void vmco_set_quantum_force(vmco_walker* w, const double* qf) { memcpy(w->qf, qf, sizeof(double) * w->ND); }
 
+/* drift limit of Umrigar, Nightingale and Runge (J. Chem. Phys. 99, 2865 (1993)):
+ * velocity v = F/2 -> v (sqrt(1 + 2 a v^2 dt) - 1) / (a v^2 dt), written stably */
+static double limited_drift(double F, double a, double dt)
+{
+    if (!(a > 0)) return F;
+    return 2.0 * F / (1.0 + sqrt(1.0 + 0.5 * a * F * F * dt));
+}
+
 int vmco_step(vmco_walker* w, int k, double z, double u_move, double u_acc)
 {
     const vmco_config* c = &w->c;
@@ -901,11 +910,11 @@ int vmco_step(vmco_walker* w, int k, double z, double u_move, double u_acc)
     /* exact Metropolis-Hastings Langevin: single-coordinate proposal
      * x_k' = x_k + dt/2 F_k(x) + sqrt(dt) z, acceptance
      * |psi'/psi|^2 exp(-dx (F+F')/2 + dt/8 (F^2 - F'^2)) */
-    double F = quantum_force(w, k);
+    double F = limited_drift(quantum_force(w, k), c->drift_limit, dt);
     double dx = 0.5 * dt * F + z * sqrt(dt);
     vmco_propose(w, k, w->x[k] + dx);
     double p = vmco_probability_ratio(w);
-    double Fn = quantum_force(w, k);
+    double Fn = limited_drift(quantum_force(w, k), c->drift_limit, dt);
     double lg = -0.5 * dx * (F + Fn) + dt / 8.0 * (F * F - Fn * Fn);
     double a = p * exp(lg);
     if (a < u_acc) { vmco_reject(w); return 0; }

@@ -59,6 +59,7 @@ typedef struct vmco_config {
     double step_length;  /* dt (importance) or brute-force step */
     unsigned quirks;     /* VMCO_Q* bits */
     int basis_scaled;    /* 0: reference Hermite H_n(x); 1: H_n(sqrt(omega) x) */
+    double drift_limit;  /* exact Langevin: F -> 2F/(1 + sqrt(1 + a F^2 dt/2)); 0 = none */
 } vmco_config;
 
 typedef struct vmco_walker vmco_walker;

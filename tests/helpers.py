@@ -1,0 +1,78 @@
+"""Shared helpers: matching oracle and engine configurations."""
+import numpy as np
+
+from oracle import oracle as O
+
+# closed-shell particle counts in 2-D and the frequencies used for them
+CLOSED_SHELLS = [2, 6, 12, 20, 30, 42, 56]
+OMEGA_FOR_N = {2: 1.0, 6: 1.0, 12: 1.0, 20: 0.5, 30: 0.5, 42: 0.28, 56: 0.1}
+
+
+def zero_variance_energy(N, omega):
+    """Non-interacting closed shells: E0 = 2 sum_{s<S} (s+1)^2 omega."""
+    E, s, filled = 0.0, 0, 0
+    while filled < N:
+        E += 2 * (s + 1) * (s + 1) * omega
+        filled += 2 * (s + 1)
+        s += 1
+    assert filled == N
+    return E
+
+
+def oracle_config(N, omega, kinds, sampling, dt, interaction=True, quirks=O.Q2 | O.Q3,
+                  drift_limit=0.0):
+    return O.make_config(N, 2, omega, int(interaction), kinds, None, None, False, sampling, dt,
+                         quirks=quirks, basis_scaled=False, drift_limit=drift_limit)
+
+
+def engine_config(N, omega, kinds, sampling, dt, interaction=True, quirks=0b1100,
+                  precision=64, walkers=64, refresh=0, drift_limit=0.0):
+    import vmachine_amd as V
+    return V.make_config(N, omega=omega, interaction=interaction, kinds=kinds, sampling=sampling,
+                         step_length=dt, quirks=quirks, precision=precision,
+                         refresh_interval=refresh, num_walkers=walkers, drift_limit=drift_limit)
+
+
+def random_parameters(rng, kinds):
+    P = np.zeros((len(kinds), 1))
+    for e, k in enumerate(kinds):
+        if k == O.GAUSSIAN:
+            P[e, 0] = 1.0 + 0.1 * rng.standard_normal()
+        elif k == O.PADE:
+            P[e, 0] = 0.3 + 0.7 * rng.random()
+    return P
+
+
+def oracle_evaluate(ocfg, P, positions, moves, new_values):
+    """The oracle's version of vmc_evaluate: rows [E, T, Vext, Vint, Ga, Gg, rS, rP, rG, drift...]."""
+    kinds = [ocfg.kinds[i] for i in range(ocfg.num_elements)]
+    W, nd = positions.shape
+    out0 = np.zeros((W, 9 + nd))
+    out1 = np.zeros((W, 9 + nd))
+
+    def fill(w, row):
+        T, V, I = w.kinetic(), w.external(), w.interaction()
+        row[0:4] = [T + V + I, T, V, I]
+        for e, k in enumerate(kinds):
+            if k == O.GAUSSIAN:
+                row[4] = w.parameter_gradient(e)[0]
+            if k == O.PADE:
+                row[5] = w.parameter_gradient(e)[0]
+        row[9:] = w.drift()
+
+    for i in range(W):
+        w = O.Walker(ocfg, P, positions[i])
+        fill(w, out0[i])
+        w.propose(int(moves[i]), float(new_values[i]))
+        for e, k in enumerate(kinds):
+            col = {O.SLATER: 6, O.PADE: 7, O.GAUSSIAN: 8}[k]
+            out1[i, col] = w.ratio(e)
+        w.accept()
+        fill(w, out1[i])
+        if O.SLATER not in kinds:
+            out1[i, 6] = 1.0
+        if O.PADE not in kinds:
+            out1[i, 7] = 1.0
+        if O.GAUSSIAN not in kinds:
+            out1[i, 8] = 1.0
+    return out0, out1

@@ -1,0 +1,50 @@
+"""The C ABI library loads without a GPU, exports everything include/vmc.h
+declares, and rejects invalid configurations before touching a device."""
+import ctypes
+
+import pytest
+
+import vmachine_amd as V
+from vmachine_amd import engine as E
+
+
+def test_library_exports_every_declared_symbol():
+    L = E.lib()
+    names = E.header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), f"libvmc_hip.so does not export {n}"
+
+
+def test_parameter_shapes_match_reference():
+    c = V.make_config(56, omega=0.1)
+    L = E.lib()
+    assert [L.vmc_num_parameters(ctypes.byref(c), e) for e in range(3)] == [0, 1, 1]
+    assert L.vmc_max_parameters(ctypes.byref(c)) == 1
+    c = V.make_config(12, kinds=[V.SLATER, V.RBM_GAUSSIAN, V.RBM_PRODUCT, V.PADE], num_hidden=24)
+    assert [L.vmc_num_parameters(ctypes.byref(c), e) for e in range(4)] == [0, 24, 600, 1]
+    assert L.vmc_max_parameters(ctypes.byref(c)) == 600
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(num_particles=8), "closed-shell"),
+    (dict(num_particles=6, num_dimensions=3), "numDimensions"),
+    (dict(num_particles=6, precision=16), "precision"),
+    (dict(num_particles=6, num_walkers=0), "num_walkers"),
+    (dict(num_particles=6, omega=-1.0), "omega"),
+    (dict(num_particles=6, sampling=7), "sampling"),
+])
+def test_invalid_configs_are_rejected(kw, msg):
+    args = dict(num_particles=6)
+    args.update(kw)
+    n = args.pop("num_particles")
+    c = V.make_config(n, **args)
+    with pytest.raises(V.VmcError, match=msg):
+        V.Engine(c)
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    if V.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(V.VmcError, match="no HIP device"):
+        V.Engine(V.make_config(6))

@@ -1,0 +1,10 @@
+"""MI355X-native VMC sampling engine for VMaChine's hot path.
+
+The compute path is libvmc_hip.so (hand-written HIP kernels for gfx950 behind
+the C ABI in include/vmc.h); vmachine_amd.engine is its ctypes binding.
+"""
+from .engine import (  # noqa: F401
+    BRUTE_FORCE, GAUSSIAN, IMPORTANCE, IMPORTANCE_EXACT, PADE, QUIRKS_REFERENCE,
+    RBM_GAUSSIAN, RBM_PRODUCT, SLATER, Engine, VmcError, device_count, make_config,
+    max_parameters,
+)

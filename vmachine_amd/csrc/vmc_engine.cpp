@@ -1,0 +1,516 @@
+// vmc_engine.cpp -- host side of the C ABI in include/vmc.h.
+//
+// Owns the device buffers of one engine (walker positions, stored drifts,
+// per-walker accumulators) and one HIP stream, validates configurations, maps
+// the reference's parameter matrix onto kernel arguments and splits a sampling
+// iteration into launches of the walk kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vmc.h"
+#include "vmc_device.h"
+
+namespace vmc {
+hipError_t launch_walk(int precision, int np, const WalkParams& P, hipStream_t stream);
+hipError_t launch_eval(int precision, int np, const WalkParams& P, const double* pos,
+                       const int* moves, const double* newv, double* o0, double* o1,
+                       hipStream_t stream);
+hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
+hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
+                                 hipStream_t stream);
+hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int nd,
+                        int n, int* k, double* z, double* um, double* ua, hipStream_t stream);
+int walkers_per_wave(int np);
+}  // namespace vmc
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(VMC_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+const int kNP[] = {1, 3, 6, 10, 15, 21, 28};
+
+// closed-shell 2-D states per spin: 2 * C(c+2, 2) == N (ref src/Basis/basis.cpp:106-123)
+int closed_shell_orbitals(int N)
+{
+    for (int c = 0; c < 16; c++) {
+        int b = (c + 1) * (c + 2) / 2;
+        if (2 * b == N) return b;
+        if (2 * b > N) return -1;
+    }
+    return -1;
+}
+
+bool has(const vmc_config* c, int kind)
+{
+    for (int e = 0; e < c->num_elements; e++)
+        if (c->kinds[e] == kind) return true;
+    return false;
+}
+
+int element_row(const vmc_config* c, int kind)
+{
+    for (int e = 0; e < c->num_elements; e++)
+        if (c->kinds[e] == kind) return e;
+    return -1;
+}
+
+}  // namespace
+
+struct vmc_engine {
+    vmc_config cfg;
+    int np = 0;             // particles per ownership slot (Slater orbitals per spin)
+    int nd = 0;
+    int maxpar = 0;
+    int row_gauss = -1, row_pade = -1;
+    double alpha = 1.0, gamma = 1.0;
+    bool params_set = false;
+    bool walkers_set = false;
+    bool init_qf = true;
+    uint64_t counter = 0;
+    uint64_t seed = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double* d_pos = nullptr;
+    double* d_qf = nullptr;
+    double* d_acc = nullptr;
+    double* d_out = nullptr;
+    float* d_rec = nullptr;
+    size_t rec_cap = 0;
+
+    ~vmc_engine()
+    {
+        if (d_pos) (void)hipFree(d_pos);
+        if (d_qf) (void)hipFree(d_qf);
+        if (d_acc) (void)hipFree(d_acc);
+        if (d_out) (void)hipFree(d_out);
+        if (d_rec) (void)hipFree(d_rec);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+const char* vmc_last_error(void) { return g_err.c_str(); }
+
+int vmc_num_parameters(const vmc_config* c, int e)
+{
+    if (!c || e < 0 || e >= c->num_elements) return 0;
+    const int ND = c->num_particles * c->num_dimensions;
+    switch (c->kinds[e]) {
+    case VMC_GAUSSIAN: return 1;
+    case VMC_SLATER: return 0;
+    case VMC_PADE: return 1;
+    case VMC_RBM_GAUSSIAN: return ND;
+    case VMC_RBM_PRODUCT: return c->num_hidden * ND + c->num_hidden;
+    }
+    return 0;
+}
+
+int vmc_max_parameters(const vmc_config* c)
+{
+    int m = 0;
+    for (int e = 0; c && e < c->num_elements; e++) {
+        const int p = vmc_num_parameters(c, e);
+        if (p > m) m = p;
+    }
+    return m;
+}
+
+int vmc_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
+{
+    if (!cfg || !out) return fail(VMC_E_INVALID, "null argument");
+    *out = nullptr;
+    const vmc_config& c = *cfg;
+    if (c.num_dimensions != 2)
+        return fail(VMC_E_INVALID, "the engine supports numDimensions: 2");
+    if (c.num_particles < 1 || c.num_particles > 56)
+        return fail(VMC_E_INVALID, "numParticles must be in 1..56");
+    if (c.num_elements < 1 || c.num_elements > 6)
+        return fail(VMC_E_INVALID, "between 1 and 6 wave-function elements");
+    for (int e = 0; e < c.num_elements; e++) {
+        const int k = c.kinds[e];
+        if (k == VMC_RBM_GAUSSIAN || k == VMC_RBM_PRODUCT)
+            return fail(VMC_E_INVALID, "RBM elements are not available in the walk kernel yet");
+        if (k != VMC_GAUSSIAN && k != VMC_SLATER && k != VMC_PADE)
+            return fail(VMC_E_INVALID, "unknown wave-function element");
+        for (int f = 0; f < e; f++)
+            if (c.kinds[f] == k) return fail(VMC_E_INVALID, "an element appears twice");
+    }
+    if (c.sampling < VMC_BRUTE_FORCE || c.sampling > VMC_IMPORTANCE_EXACT)
+        return fail(VMC_E_INVALID, "unknown sampling method");
+    if (c.precision != 32 && c.precision != 64)
+        return fail(VMC_E_INVALID, "precision must be 32 or 64");
+    if (c.num_walkers < 1) return fail(VMC_E_INVALID, "num_walkers must be positive");
+    if (!(c.omega > 0)) return fail(VMC_E_INVALID, "omega must be positive");
+    if (!(c.step_length > 0)) return fail(VMC_E_INVALID, "stepLength must be positive");
+    if (c.refresh_interval < 0) return fail(VMC_E_INVALID, "refresh_interval must be >= 0");
+    if (!(c.drift_limit >= 0)) return fail(VMC_E_INVALID, "drift_limit must be >= 0");
+
+    int np;
+    if (has(&c, VMC_SLATER)) {
+        np = closed_shell_orbitals(c.num_particles);
+        if (np < 0)
+            return fail(VMC_E_INVALID, "slaterDeterminant needs a closed-shell numParticles "
+                                       "(2, 6, 12, 20, 30, 42, 56)");
+    } else {
+        const int half = (c.num_particles + 1) / 2;
+        np = 0;
+        for (int v : kNP)
+            if (v >= half) { np = v; break; }
+    }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(VMC_E_DEVICE, "no HIP device visible");
+
+    vmc_engine* e = new vmc_engine();
+    e->cfg = c;
+    e->np = np;
+    e->nd = 2 * c.num_particles;
+    e->maxpar = vmc_max_parameters(&c);
+    e->row_gauss = element_row(&c, VMC_GAUSSIAN);
+    e->row_pade = element_row(&c, VMC_PADE);
+    auto cleanup = [&](int code) {
+        delete e;
+        return code;
+    };
+    if (hipGetDevice(&e->device) != hipSuccess) return cleanup(fail(VMC_E_DEVICE, "hipGetDevice"));
+    const size_t W = (size_t)c.num_walkers;
+    hipError_t err;
+    if ((err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (err = hipEventCreate(&e->ev0)) != hipSuccess ||
+        (err = hipEventCreate(&e->ev1)) != hipSuccess ||
+        (err = hipMalloc(&e->d_pos, W * e->nd * sizeof(double))) != hipSuccess ||
+        (err = hipMalloc(&e->d_qf, W * e->nd * sizeof(double))) != hipSuccess ||
+        (err = hipMalloc(&e->d_acc, W * vmc::kNumWalkerAcc * sizeof(double))) != hipSuccess ||
+        (err = hipMalloc(&e->d_out, vmc::kNumWalkerAcc * sizeof(double))) != hipSuccess)
+        return cleanup(fail(VMC_E_DEVICE, std::string("allocation: ") + hipGetErrorString(err)));
+    *out = e;
+    g_err.clear();
+    return VMC_OK;
+}
+
+void vmc_engine_destroy(vmc_engine* e) { delete e; }
+
+int vmc_set_parameters(vmc_engine* e, const double* params, int rows, int cols)
+{
+    if (!e || !params) return fail(VMC_E_INVALID, "null argument");
+    if (rows != e->cfg.num_elements || cols != e->maxpar)
+        return fail(VMC_E_INVALID, "parameter matrix must be num_elements x max_parameters");
+    if (e->row_gauss >= 0) e->alpha = params[e->row_gauss * cols + 0];
+    if (e->row_pade >= 0) e->gamma = params[e->row_pade * cols + 0];
+    e->params_set = true;
+    return VMC_OK;
+}
+
+int vmc_init_walkers(vmc_engine* e, uint64_t seed)
+{
+    if (!e) return fail(VMC_E_INVALID, "null engine");
+    HIP_TRY(vmc::launch_init_positions(e->d_pos, e->cfg.num_walkers, e->nd, e->cfg.omega, seed,
+                                       e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->seed = seed;
+    e->counter = 0;
+    e->init_qf = true;
+    e->walkers_set = true;
+    return VMC_OK;
+}
+
+static int check_walkers(vmc_engine* e, const void* p, int walkers, int dof)
+{
+    if (!e || !p) return fail(VMC_E_INVALID, "null argument");
+    if (walkers != e->cfg.num_walkers || dof != e->nd)
+        return fail(VMC_E_INVALID, "expected num_walkers x (num_particles*2) values");
+    return VMC_OK;
+}
+
+int vmc_set_positions(vmc_engine* e, const double* pos, int walkers, int dof)
+{
+    if (int rc = check_walkers(e, pos, walkers, dof)) return rc;
+    HIP_TRY(hipMemcpyAsync(e->d_pos, pos, sizeof(double) * walkers * dof, hipMemcpyHostToDevice,
+                           e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->walkers_set = true;
+    e->init_qf = true;
+    return VMC_OK;
+}
+
+int vmc_get_positions(vmc_engine* e, double* pos, int walkers, int dof)
+{
+    if (int rc = check_walkers(e, pos, walkers, dof)) return rc;
+    HIP_TRY(hipMemcpyAsync(pos, e->d_pos, sizeof(double) * walkers * dof, hipMemcpyDeviceToHost,
+                           e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return VMC_OK;
+}
+
+int vmc_set_quantum_force(vmc_engine* e, const double* qf, int walkers, int dof)
+{
+    if (int rc = check_walkers(e, qf, walkers, dof)) return rc;
+    HIP_TRY(hipMemcpyAsync(e->d_qf, qf, sizeof(double) * walkers * dof, hipMemcpyHostToDevice,
+                           e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->init_qf = false;
+    return VMC_OK;
+}
+
+int vmc_get_quantum_force(vmc_engine* e, double* qf, int walkers, int dof)
+{
+    if (int rc = check_walkers(e, qf, walkers, dof)) return rc;
+    HIP_TRY(hipMemcpyAsync(qf, e->d_qf, sizeof(double) * walkers * dof, hipMemcpyDeviceToHost,
+                           e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return VMC_OK;
+}
+
+int vmc_set_random_state(vmc_engine* e, uint64_t seed, uint64_t counter)
+{
+    if (!e) return fail(VMC_E_INVALID, "null engine");
+    e->seed = seed;
+    e->counter = counter;
+    return VMC_OK;
+}
+
+int vmc_get_random_state(vmc_engine* e, uint64_t* seed, uint64_t* counter)
+{
+    if (!e || !seed || !counter) return fail(VMC_E_INVALID, "null argument");
+    *seed = e->seed;
+    *counter = e->counter;
+    return VMC_OK;
+}
+
+static vmc::WalkParams base_params(const vmc_engine* e)
+{
+    const vmc_config& c = e->cfg;
+    vmc::WalkParams P{};
+    P.N = c.num_particles;
+    P.n_up = c.num_particles / 2;
+    P.omega = c.omega;
+    P.has_gauss = e->row_gauss >= 0;
+    P.has_slater = has(&c, VMC_SLATER);
+    P.has_pade = e->row_pade >= 0;
+    P.coulomb = c.interaction ? 1 : 0;
+    P.alpha = e->alpha;
+    P.gamma = e->gamma;
+    P.a_same = 1.0 / 3.0;   // 2-D spin-parallel Pade a (ref padejastrow.cpp:143-146)
+    P.a_anti = 1.0;
+    P.sampling = c.sampling;
+    P.dt = c.step_length;
+    P.q2_upper = (c.quirks & VMC_Q2_PADE_RATIO_UPPER) ? 1 : 0;
+    P.q3_half = (c.quirks & VMC_Q3_PADE_GAMMA_HALF) ? 1 : 0;
+    P.drift_a = c.drift_limit;
+    P.seed = e->seed;
+    P.W = c.num_walkers;
+    return P;
+}
+
+static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batches,
+                    vmc_sums* sums, double* grad, double* grad_e, int rec_walkers, float* energies)
+{
+    if (!e) return fail(VMC_E_INVALID, "null engine");
+    if (!e->params_set) return fail(VMC_E_STATE, "vmc_set_parameters has not been called");
+    if (!e->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
+    if (steps < 1 || equil < 0 || batches < 1 || iteration < 0)
+        return fail(VMC_E_INVALID, "steps >= 1, equilibration >= 0, batches >= 1 required");
+    if (rec_walkers < 0 || rec_walkers > e->cfg.num_walkers)
+        return fail(VMC_E_INVALID, "record_walkers out of range");
+    const int W = e->cfg.num_walkers;
+    if (rec_walkers > 0) {
+        const size_t need = (size_t)rec_walkers * steps;
+        if (need > e->rec_cap) {
+            if (e->d_rec) (void)hipFree(e->d_rec);
+            e->d_rec = nullptr;
+            e->rec_cap = 0;
+            HIP_TRY(hipMalloc(&e->d_rec, need * sizeof(float)));
+            e->rec_cap = need;
+        }
+    }
+    vmc::WalkParams P = base_params(e);
+    P.counter0 = e->counter;
+    P.steps = steps;
+    P.equil = equil;
+    const int S = steps / batches;                 // ref sampler.cpp:104
+    const int kb = iteration % batches;
+    P.window_lo = S * kb;
+    P.window_hi = S * (kb + 1);
+    P.pos = e->d_pos;
+    P.qf = (e->cfg.sampling == VMC_IMPORTANCE) ? e->d_qf : nullptr;
+    P.acc = e->d_acc;
+    P.rec = rec_walkers > 0 ? e->d_rec : nullptr;
+    P.rec_walkers = rec_walkers;
+    P.init_qf = e->init_qf ? 1 : 0;
+    const int total = equil + steps;
+    const int seg = (e->cfg.refresh_interval > 0) ? e->cfg.refresh_interval : total;
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    for (int t0 = 0; t0 < total; t0 += seg) {
+        P.t_begin = t0;
+        P.t_end = (t0 + seg < total) ? t0 + seg : total;
+        P.acc_add = t0 > 0 ? 1 : 0;
+        HIP_TRY(vmc::launch_walk(e->cfg.precision, e->np, P, e->stream));
+    }
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    HIP_TRY(vmc::launch_reduce(e->d_acc, W, e->d_out, e->stream));
+    double out[vmc::kNumWalkerAcc];
+    HIP_TRY(hipMemcpyAsync(out, e->d_out, sizeof(out), hipMemcpyDeviceToHost, e->stream));
+    if (rec_walkers > 0 && energies)
+        HIP_TRY(hipMemcpyAsync(energies, e->d_rec, sizeof(float) * (size_t)rec_walkers * steps,
+                               hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->counter += (uint64_t)total;
+    e->init_qf = false;
+    if (sums) {
+        sums->energy = out[vmc::kAccE];
+        sums->energy2 = out[vmc::kAccE2];
+        sums->kinetic = out[vmc::kAccT];
+        sums->external = out[vmc::kAccVext];
+        sums->interaction = out[vmc::kAccVint];
+        sums->kinetic2 = out[vmc::kAccT2];
+        sums->external2 = out[vmc::kAccVext2];
+        sums->interaction2 = out[vmc::kAccVint2];
+        sums->accepted = out[vmc::kAccAccept];
+        sums->samples = (double)W * steps;
+        sums->gradient_samples = (double)W * S;
+        sums->kernel_ms = ms;
+        sums->nonfinite = out[vmc::kAccBad];
+    }
+    const int E = e->cfg.num_elements, mp = e->maxpar;
+    for (double* g : {grad, grad_e}) {
+        if (!g) continue;
+        for (int i = 0; i < E * mp; i++) g[i] = 0.0;
+    }
+    if (mp > 0) {
+        if (e->row_gauss >= 0) {
+            if (grad) grad[e->row_gauss * mp] = out[vmc::kAccGAlpha];
+            if (grad_e) grad_e[e->row_gauss * mp] = out[vmc::kAccGAlphaE];
+        }
+        if (e->row_pade >= 0) {
+            if (grad) grad[e->row_pade * mp] = out[vmc::kAccGGamma];
+            if (grad_e) grad_e[e->row_pade * mp] = out[vmc::kAccGGammaE];
+        }
+    }
+    return VMC_OK;
+}
+
+int vmc_run(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+            vmc_sums* sums, double* grad, double* grad_e)
+{
+    return run_impl(e, steps, equilibration, iteration, batches, sums, grad, grad_e, 0, nullptr);
+}
+
+int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+                     vmc_sums* sums, double* grad, double* grad_e, int record_walkers,
+                     float* energies)
+{
+    if (record_walkers > 0 && !energies) return fail(VMC_E_INVALID, "null energies buffer");
+    return run_impl(e, steps, equilibration, iteration, batches, sums, grad, grad_e,
+                    record_walkers, energies);
+}
+
+int vmc_evaluate(vmc_engine* e, const double* positions, int walkers, const int* moves,
+                 const double* new_values, double* out_before, double* out_after)
+{
+    if (!e || !positions || !moves || !new_values || !out_before || !out_after)
+        return fail(VMC_E_INVALID, "null argument");
+    if (!e->params_set) return fail(VMC_E_STATE, "vmc_set_parameters has not been called");
+    if (walkers < 1) return fail(VMC_E_INVALID, "walkers must be positive");
+    for (int w = 0; w < walkers; w++)
+        if (moves[w] < 0 || moves[w] >= e->nd) return fail(VMC_E_INVALID, "move index out of range");
+    const int width = 9 + e->nd;
+    double *d_p = nullptr, *d_v = nullptr, *d_o0 = nullptr, *d_o1 = nullptr;
+    int* d_m = nullptr;
+    auto release = [&]() {
+        if (d_p) (void)hipFree(d_p);
+        if (d_v) (void)hipFree(d_v);
+        if (d_o0) (void)hipFree(d_o0);
+        if (d_o1) (void)hipFree(d_o1);
+        if (d_m) (void)hipFree(d_m);
+    };
+    hipError_t err;
+    vmc::WalkParams P = base_params(e);
+    P.W = walkers;
+    if ((err = hipMalloc(&d_p, sizeof(double) * walkers * e->nd)) != hipSuccess ||
+        (err = hipMalloc(&d_v, sizeof(double) * walkers)) != hipSuccess ||
+        (err = hipMalloc(&d_m, sizeof(int) * walkers)) != hipSuccess ||
+        (err = hipMalloc(&d_o0, sizeof(double) * walkers * width)) != hipSuccess ||
+        (err = hipMalloc(&d_o1, sizeof(double) * walkers * width)) != hipSuccess ||
+        (err = hipMemcpyAsync(d_p, positions, sizeof(double) * walkers * e->nd,
+                              hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(d_v, new_values, sizeof(double) * walkers, hipMemcpyHostToDevice,
+                              e->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(d_m, moves, sizeof(int) * walkers, hipMemcpyHostToDevice,
+                              e->stream)) != hipSuccess ||
+        (err = vmc::launch_eval(e->cfg.precision, e->np, P, d_p, d_m, d_v, d_o0, d_o1,
+                                e->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(out_before, d_o0, sizeof(double) * walkers * width,
+                              hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(out_after, d_o1, sizeof(double) * walkers * width,
+                              hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
+        (err = hipStreamSynchronize(e->stream)) != hipSuccess) {
+        release();
+        return fail(VMC_E_DEVICE, std::string("evaluate: ") + hipGetErrorString(err));
+    }
+    release();
+    return VMC_OK;
+}
+
+int vmc_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int n, int dof,
+              int* coords, double* normals, double* u_move, double* u_accept)
+{
+    if (!coords || !normals || !u_move || !u_accept || n < 1 || dof < 1)
+        return fail(VMC_E_INVALID, "bad argument");
+    if (precision != 32 && precision != 64) return fail(VMC_E_INVALID, "precision must be 32 or 64");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(VMC_E_DEVICE, "no HIP device visible");
+    int* d_k = nullptr;
+    double* d_f = nullptr;
+    hipError_t err;
+    if ((err = hipMalloc(&d_k, sizeof(int) * n)) != hipSuccess ||
+        (err = hipMalloc(&d_f, sizeof(double) * 3 * n)) != hipSuccess ||
+        (err = vmc::launch_draws(precision, seed, walker, counter0, dof, n, d_k, d_f, d_f + n,
+                                 d_f + 2 * n, nullptr)) != hipSuccess ||
+        (err = hipMemcpy(coords, d_k, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (err = hipMemcpy(normals, d_f, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (err = hipMemcpy(u_move, d_f + n, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (err = hipMemcpy(u_accept, d_f + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost)) !=
+            hipSuccess) {
+        if (d_k) (void)hipFree(d_k);
+        if (d_f) (void)hipFree(d_f);
+        return fail(VMC_E_DEVICE, std::string("draws: ") + hipGetErrorString(err));
+    }
+    (void)hipFree(d_k);
+    (void)hipFree(d_f);
+    return VMC_OK;
+}
+
+}  // extern "C"

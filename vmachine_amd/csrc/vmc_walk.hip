@@ -1,0 +1,977 @@
+// vmc_walk.hip -- the persistent Metropolis walk kernel for gfx950 (MI355X).
+//
+// One walker is a group of K lanes of a 64-lane wavefront (K = the smallest
+// power of two >= NP, NP = particles per spin block), so a wave carries 64/K
+// walkers.  Lane l of a walker owns
+//   * particle l (slot 0) and particle NP+l (slot 1): positions, Slater
+//     gradient and second-derivative sums, Pade gradient, stale drift;
+//   * column l of the spin-up and of the spin-down Slater inverse, held in
+//     registers for the whole launch (so a Sherman-Morrison update, a ratio
+//     and the derivative refresh of particle l touch no memory at all).
+// Pair terms of a moved particle are lane-parallel over the other particles;
+// sums over a walker are butterfly reductions inside the K-lane group.  The
+// only HBM traffic is the walker's positions at launch start and end and one
+// row of per-walker accumulators.
+//
+// The walk reproduces, per step, the reference's single-coordinate Metropolis
+// step and its per-step sampling:
+//   ref src/system.cpp:147-164            System::runMetropolisCycles
+//   ref src/Metropolis/importancesampling.cpp:34-81, bruteforce.cpp:26-52
+//   ref src/elementoperation.cpp:80-123   ratio, kinetic energy, parameter gradients
+//   ref src/WaveFunctions/gaussian.cpp, slaterdeterminant.cpp, padejastrow.cpp
+//   ref src/Hamiltonians/harmonicoscillator.cpp:16-20, src/Interaction/coulomb.cpp:17-48
+//   ref src/sampler.cpp:52-90              per-step accumulation and gradient window
+// The Slater inverse is built in FP64 by Gauss-Jordan elimination with partial
+// pivoting at the start of every launch (the host splits an iteration into
+// launches of `refresh` steps); in between it is kept by Sherman-Morrison
+// updates in the walk precision R.  The orbitals are
+// the normalised Hermite polynomials h_n(sqrt(omega) x): for closed shells they
+// span the same polynomial space as the reference's H_n(x) (ref
+// src/Basis/hermite.cpp), so every ratio and log-derivative is identical, and
+// they stay well conditioned at small omega.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "vmc_device.h"
+
+namespace vmc {
+
+// ------------------------------------------------------------------------
+// compile-time tables
+// ------------------------------------------------------------------------
+constexpr double csqrt_iter(double x, double cur, double prev)
+{
+    return cur == prev ? cur : csqrt_iter(x, 0.5 * (cur + x / cur), cur);
+}
+constexpr double csqrt(double x) { return x <= 0 ? 0.0 : csqrt_iter(x, x > 1 ? x : 1.0, 0.0); }
+
+struct HermiteConst {
+    double c1[8];  // sqrt(2/(m+1))          h_{m+1} = c1 t h_m - c2 h_{m-1}
+    double c2[8];  // sqrt(m/(m+1))
+    double d1[8];  // sqrt(2m):      h_m'  = d1 h_{m-1}
+    double d2[8];  // 2 sqrt(m(m-1)): h_m'' = d2 h_{m-2}
+};
+constexpr HermiteConst make_hermite_const()
+{
+    HermiteConst h{};
+    for (int m = 0; m < 8; m++) {
+        h.c1[m] = csqrt(2.0 / (m + 1));
+        h.c2[m] = csqrt(double(m) / (m + 1));
+        h.d1[m] = csqrt(2.0 * m);
+        h.d2[m] = 2.0 * csqrt(double(m) * (m - 1 > 0 ? m - 1 : 0));
+    }
+    return h;
+}
+constexpr HermiteConst kHC = make_hermite_const();
+
+// closed-shell 2-D orbital list: shell i holds (n_x, n_y) = (i-j, j), j = 0..i
+// (ref src/Basis/basis.cpp:125-143)
+constexpr int orb_shell(int k)
+{
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= k) i++;
+    return i;
+}
+constexpr int orb_ny(int k) { return k - orb_shell(k) * (orb_shell(k) + 1) / 2; }
+constexpr int orb_nx(int k) { return orb_shell(k) - orb_ny(k); }
+constexpr int max_degree(int np) { return np <= 1 ? 0 : orb_shell(np - 1); }
+
+// ------------------------------------------------------------------------
+// precision-dependent math
+// ------------------------------------------------------------------------
+template <typename R> struct M;
+template <> struct M<float> {
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+    static __device__ __forceinline__ float log(float x) { return __logf(x); }
+    // v_cos_f32 takes its argument in revolutions: cos(2 pi u)
+    static __device__ __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
+};
+template <> struct M<double> {
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ double cos2pi(double u) { return ::cos(6.283185307179586476925286766559 * u); }
+};
+
+// normalised Hermite values h_0..h_S at t
+template <typename R, int S>
+__device__ __forceinline__ void hermite(R t, R (&h)[S + 1])
+{
+    h[0] = R(1);
+#pragma unroll
+    for (int m = 0; m < S; m++)
+        h[m + 1] = R(kHC.c1[m]) * t * h[m] - (m ? R(kHC.c2[m]) * h[m - 1] : R(0));
+}
+
+// ------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and the engine's draw protocol
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
+
+// 23-bit uniform in (0,1): (m + 1/2) 2^-23 with m < 2^23 needs 24 significant
+// bits, so it is exact in float and double alike
+template <typename R>
+__device__ __forceinline__ R u23(uint32_t u)
+{
+    return (R(u >> 9) + R(0.5)) * R(1.0 / 8388608.0);
+}
+
+// step draws: coordinate, standard normal (Box-Muller), move uniform, accept uniform
+template <typename R>
+__device__ __forceinline__ void step_draws(uint64_t seed, uint32_t walker, uint64_t counter,
+                                           int nd, int& k, R& z, R& um, R& ua)
+{
+    uint32_t c[4] = {(uint32_t)counter, (uint32_t)(counter >> 32), walker, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    k = (int)__umulhi(c[0], (uint32_t)nd);
+    const R U1 = u23<R>(c[1]), U2 = u23<R>(c[2]);
+    z = M<R>::sqrt(R(-2) * M<R>::log(U1)) * M<R>::cos2pi(U2);
+    um = U1;
+    ua = u23<R>(c[3]);
+}
+
+// ------------------------------------------------------------------------
+// K-lane group collectives
+// ------------------------------------------------------------------------
+template <int K, typename T>
+__device__ __forceinline__ T gsum(T v)
+{
+#pragma unroll
+    for (int m = K / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <int K>
+__device__ __forceinline__ void gargmax(double& v, int& idx)
+{
+#pragma unroll
+    for (int m = K / 2; m >= 1; m >>= 1) {
+        const double ov = __shfl_xor(v, m, 64);
+        const int oi = __shfl_xor(idx, m, 64);
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T vsel(bool c, T a, T b) { return c ? a : b; }
+
+// compile-time loop: f(std::integral_constant<int, i>) for i = B..E-1, so that
+// register arrays are only ever indexed by constants
+template <int B, int E, typename F>
+__device__ __forceinline__ void sfor(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>());
+        sfor<B + 1, E>(f);
+    }
+}
+
+// ------------------------------------------------------------------------
+// walker state
+// ------------------------------------------------------------------------
+template <typename R, int NP, int K>
+struct Walker {
+    static constexpr int S = max_degree(NP);
+    static constexpr int G = 64 / K;
+
+    // owned particles: slot 0 = particle l, slot 1 = particle NP + l
+    R x[2][2];
+    R up[NP], dn[NP];      // column l of the spin-up / spin-down Slater inverse
+    R gS[2][2];            // d ln det / dx of the owned particles
+    R sS[2];               // sum_d d^2 det / dx_d^2 / det of the owned particles
+    R gJ[2][2];            // d ln J / dx of the owned particles (Pade)
+    R qf[2][2];            // stale quantum force (importance compat)
+    double sumx2, vint, lj, jg;   // walker sums: sum x^2, Coulomb, Pade Laplacian, sum a h^2
+    double el, tk, vext, ga, gg;  // local energy parts and parameter gradients of the current state
+    bool valid[2];
+    bool upspin[2];
+    int l, gbase, g;
+
+    __device__ __forceinline__ R sw(const WalkParams& P) const { return R(::sqrt(P.omega)); }
+
+    // Slater log-derivatives of particle (b*NP + l) from column `col` of its block inverse
+    __device__ __forceinline__ void slater_derivs(const WalkParams& P, int b, const R (&col)[NP],
+                                                  R& gx, R& gy, R& sec) const
+    {
+        const R s = sw(P);
+        const R t0 = s * x[b][0], t1 = s * x[b][1];
+        R hx[S + 1], hy[S + 1];
+        hermite<R, S>(t0, hx);
+        hermite<R, S>(t1, hy);
+        R u0[S + 1], u1[S + 1], u2[S + 1];
+#pragma unroll
+        for (int i = 0; i <= S; i++) u0[i] = u1[i] = u2[i] = R(0);
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            const int nx = orb_nx(k), ny = orb_ny(k);
+            const R B = col[k];
+            u0[nx] += B * hy[ny];
+            if (ny >= 1) u1[nx] += B * R(kHC.d1[ny]) * hy[ny >= 1 ? ny - 1 : 0];
+            if (ny >= 2) u2[nx] += B * R(kHC.d2[ny]) * hy[ny >= 2 ? ny - 2 : 0];
+        }
+        R ax = R(0), ay = R(0), ss = R(0);
+#pragma unroll
+        for (int i = 0; i <= S; i++) {
+            if (i >= 1) ax += R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0] * u0[i];
+            if (i >= 2) ss += R(kHC.d2[i]) * hx[i >= 2 ? i - 2 : 0] * u0[i];
+            ay += hx[i] * u1[i];
+            ss += hx[i] * u2[i];
+        }
+        gx = s * ax;
+        gy = s * ay;
+        sec = s * s * ss;
+    }
+
+    // FP64 Gauss-Jordan inverse of spin block b; leaves column l in `out`.
+    __device__ __forceinline__ void invert_block(const WalkParams& P, int b, double* lds_T, double* lds_row,
+                                 R (&out)[NP])
+    {
+        const double s = ::sqrt(P.omega);
+        double a[NP];
+        {
+            const double t0 = s * double(x[b][0]), t1 = s * double(x[b][1]);
+            double hx[S + 1], hy[S + 1];
+            hermite<double, S>(t0, hx);
+            hermite<double, S>(t1, hy);
+#pragma unroll
+            for (int k = 0; k < NP; k++) a[k] = hx[orb_nx(k)] * hy[orb_ny(k)];
+        }
+        bool used = (l >= NP);
+        int mycol = 0;
+        int rowof[NP];
+        sfor<0, NP>([&](auto ci) {
+            constexpr int c = decltype(ci)::value;
+            double v = used ? -1.0 : fabs(a[c]);
+            int idx = l;
+            gargmax<K>(v, idx);
+            rowof[c] = idx;
+            double* row = lds_row + ((c & 1) * G + g) * NP;
+            if (l == idx) {
+#pragma unroll
+                for (int j = 0; j < NP; j++) row[j] = a[j];
+            }
+            __syncthreads();
+            const double ip = 1.0 / row[c];
+            if (l == idx) {
+                used = true;
+                mycol = c;
+#pragma unroll
+                for (int j = 0; j < NP; j++) a[j] = (j == c) ? ip : a[j] * ip;
+            } else {
+                const double f = a[c] * ip;
+#pragma unroll
+                for (int j = 0; j < NP; j++) a[j] = (j == c) ? -f : a[j] - f * row[j];
+            }
+        });
+        // lane rowof[c] now holds row c of the inverse with its columns permuted:
+        // a[j] = inv[c][rowof[j]].  Transpose through LDS so lane q gets inv[.][q].
+        double* T = lds_T + g * NP * NP;
+        if (l < NP) {
+#pragma unroll
+            for (int j = 0; j < NP; j++) T[rowof[j] * NP + mycol] = a[j];
+        }
+        __syncthreads();
+        if (l < NP) {
+#pragma unroll
+            for (int k = 0; k < NP; k++) out[k] = R(T[l * NP + k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NP; k++) out[k] = R(0);
+        }
+        __syncthreads();
+    }
+
+    __device__ __forceinline__ void rebuild_slater(const WalkParams& P, double* lds_T, double* lds_row)
+    {
+        invert_block(P, 0, lds_T, lds_row, up);
+        invert_block(P, 1, lds_T, lds_row, dn);
+        R gx, gy, sec;
+        slater_derivs(P, 0, up, gx, gy, sec);
+        gS[0][0] = gx; gS[0][1] = gy; sS[0] = sec;
+        slater_derivs(P, 1, dn, gx, gy, sec);
+        gS[1][0] = gx; gS[1][1] = gy; sS[1] = sec;
+        if (l >= NP) {
+            gS[0][0] = gS[0][1] = gS[1][0] = gS[1][1] = R(0);
+            sS[0] = sS[1] = R(0);
+        }
+    }
+
+    // Pade gradients and the walker sums, from scratch (O(N^2) per walker)
+    __device__ __forceinline__ void rebuild_pairs(const WalkParams& P)
+    {
+        const R gam = R(P.gamma);
+        double vi = 0, ljs = 0, jgs = 0;
+#pragma unroll
+        for (int s = 0; s < 2; s++) gJ[s][0] = gJ[s][1] = R(0);
+        sfor<0, 2 * NP>([&](auto qi) {
+            constexpr int qq = decltype(qi)::value;
+            constexpr int own = qq % NP, sl = qq / NP;
+            const R y0 = __shfl(x[sl][0], gbase + own, 64);
+            const R y1 = __shfl(x[sl][1], gbase + own, 64);
+            if (qq >= P.N) return;
+            const bool qq_up = qq < P.n_up;
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const int q = s * NP + l;
+                if (!valid[s] || q == qq) continue;
+                const R e0 = x[s][0] - y0, e1 = x[s][1] - y1;
+                const R r = M<R>::sqrt(e0 * e0 + e1 * e1);
+                const R ir = M<R>::rcp(r);
+                const R a = (upspin[s] == qq_up) ? R(P.a_same) : R(P.a_anti);
+                const R f = M<R>::rcp(R(1) + gam * r);
+                const R u1 = a * f * f;
+                if (P.has_pade) {
+                    gJ[s][0] += u1 * e0 * ir;
+                    gJ[s][1] += u1 * e1 * ir;
+                }
+                if (qq > q) {
+                    vi += double(ir);
+                    ljs += double(R(2) * u1 * (ir - R(2) * gam * f));
+                    const R h = r * f;
+                    jgs += double(a * h * h);
+                }
+            }
+        });
+        vint = P.coulomb ? gsum<K>(vi) : 0.0;
+        lj = P.has_pade ? gsum<K>(ljs) : 0.0;
+        jg = P.has_pade ? gsum<K>(jgs) : 0.0;
+        if (!P.has_pade) {
+#pragma unroll
+            for (int s = 0; s < 2; s++) gJ[s][0] = gJ[s][1] = R(0);
+        }
+        double sx = 0;
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            if (valid[s]) sx += double(x[s][0]) * double(x[s][0]) + double(x[s][1]) * double(x[s][1]);
+        sumx2 = gsum<K>(sx);
+    }
+
+    // local energy of the current state (ref src/elementoperation.cpp:95-109, src/sampler.cpp:66-70)
+    __device__ __forceinline__ void local_energy(const WalkParams& P)
+    {
+        const R oa = R(P.omega * P.alpha);
+        double t = 0;
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            if (!valid[s]) continue;
+            // lap ln det + |grad ln psi|^2 with the |grad ln det|^2 parts cancelled
+            // analytically: sS - gS^2 + (gS + o)^2 = sS + 2 gS.o + o^2, o = the
+            // other elements' gradient.  Near a node gS ~ 1/distance, and the
+            // explicit difference would lose every digit in FP32.
+            R o0 = R(0), o1 = R(0);
+            if (P.has_pade) {
+                o0 += gJ[s][0];
+                o1 += gJ[s][1];
+            }
+            if (P.has_gauss) {
+                o0 -= oa * x[s][0];
+                o1 -= oa * x[s][1];
+            }
+            double v = double(o0) * double(o0) + double(o1) * double(o1);
+            if (P.has_slater)
+                v += double(sS[s]) + 2.0 * (double(gS[s][0]) * double(o0) +
+                                            double(gS[s][1]) * double(o1));
+            t += v;
+        }
+        t = gsum<K>(t);
+        if (P.has_pade) t += lj;
+        if (P.has_gauss) t -= P.omega * P.alpha * 2.0 * P.N;
+        tk = -0.5 * t;
+        vext = 0.5 * P.omega * P.omega * sumx2;
+        el = tk + vext + vint;
+        ga = P.has_gauss ? -0.5 * P.omega * sumx2 : 0.0;
+        gg = P.has_pade ? (P.q3_half ? -0.5 : -1.0) * jg : 0.0;
+    }
+};
+
+// ------------------------------------------------------------------------
+// one Metropolis step of a walker (ref importancesampling.cpp:49-81,
+// bruteforce.cpp:26-52, elementoperation.cpp:38-87)
+// ------------------------------------------------------------------------
+template <typename R>
+struct StepInfo {
+    R rs2, rp, rg;   // Slater |det'/det|^2, Pade and Gaussian ratios of the proposal
+};
+
+template <typename R, int NP, int K>
+__device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParams& P, int k, R z,
+                                            R um, R ua, bool forced, R xforced, StepInfo<R>& info)
+{
+    using WK = Walker<R, NP, K>;
+    const R dt = R(P.dt), sqdt = R(::sqrt(P.dt));
+    const R oa = R(P.omega * P.alpha);
+    const R gam = R(P.gamma);
+    const int p = k >> 1, d = k & 1;
+    const int slot = p >= NP ? 1 : 0;
+    const int lp = p - slot * NP;
+    const int osrc = w.gbase + lp;
+    const bool pup = p < P.n_up;
+
+    // current position and drift of the moved particle
+    // (value selects throughout: a conditional between two register-array
+    // elements must not become a select of addresses, which would force the
+    // array into scratch memory)
+    const R xp0 = __shfl(vsel(slot, w.x[1][0], w.x[0][0]), osrc, 64);
+    const R xp1 = __shfl(vsel(slot, w.x[1][1], w.x[0][1]), osrc, 64);
+    R mg = R(0);
+    if (P.has_slater) mg += vsel(slot, vsel(d, w.gS[1][1], w.gS[1][0]), vsel(d, w.gS[0][1], w.gS[0][0]));
+    if (P.has_pade) mg += vsel(slot, vsel(d, w.gJ[1][1], w.gJ[1][0]), vsel(d, w.gJ[0][1], w.gJ[0][0]));
+    const R xpd = d ? xp1 : xp0;
+    R Gd = __shfl(mg, osrc, 64);
+    if (P.has_gauss) Gd -= oa * xpd;
+    const R F = R(2) * Gd;
+    const R stale =
+        __shfl(vsel(slot, vsel(d, w.qf[1][1], w.qf[1][0]), vsel(d, w.qf[0][1], w.qf[0][0])), osrc, 64);
+
+    // drift of the proposal: the reference's F, or in exact Langevin with a drift
+    // limit a > 0, 2F / (1 + sqrt(1 + a F^2 dt / 2))
+    const bool limit = P.sampling == kImportanceExact && P.drift_a > 0;
+    const R Fp = limit ? R(2) * F / (R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * F * F * dt)) : F;
+    R dx;
+    if (forced) dx = xforced - xpd;
+    else if (P.sampling == kBruteForce) dx = R(10) * (um - R(0.5)) * dt;
+    else dx = R(0.5) * dt * Fp + sqdt * z;
+    const R xnd = forced ? xforced : xpd + dx;
+    const R xn0 = d ? xp0 : xnd;
+    const R xn1 = d ? xnd : xp1;
+
+    // pair terms of the moved particle against the owned ones
+    R jexp = R(0), dvi = R(0), dlj = R(0), djg = R(0), gp0 = R(0), gp1 = R(0);
+    R dg[2][2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int q = s * NP + w.l;
+        const bool val = w.valid[s] && q != p;
+        const R e0 = w.x[s][0] - xp0, e1 = w.x[s][1] - xp1;
+        const R n0 = w.x[s][0] - xn0, n1 = w.x[s][1] - xn1;
+        const R ro = M<R>::sqrt(e0 * e0 + e1 * e1);
+        const R rn = M<R>::sqrt(n0 * n0 + n1 * n1);
+        const R iro = val ? M<R>::rcp(ro) : R(0);
+        const R irn = val ? M<R>::rcp(rn) : R(0);
+        const R a = (w.upspin[s] == pup) ? R(P.a_same) : R(P.a_anti);
+        const R fo = M<R>::rcp(R(1) + gam * ro), fn = M<R>::rcp(R(1) + gam * rn);
+        const R ho = ro * fo, hn = rn * fn;
+        const R uo = a * fo * fo, un = a * fn * fn;
+        const bool in_ratio = val && (!P.q2_upper || q > p);
+        jexp += in_ratio ? a * (hn - ho) : R(0);
+        dvi += irn - iro;
+        dlj += val ? R(2) * (un * (irn - R(2) * gam * fn) - uo * (iro - R(2) * gam * fo)) : R(0);
+        djg += val ? a * (hn * hn - ho * ho) : R(0);
+        gp0 -= un * n0 * irn;
+        gp1 -= un * n1 * irn;
+        dg[s][0] = un * n0 * irn - uo * e0 * iro;
+        dg[s][1] = un * n1 * irn - uo * e1 * iro;
+    }
+    if (P.has_pade) {
+        jexp = gsum<K>(jexp);
+        dlj = gsum<K>(dlj);
+        djg = gsum<K>(djg);
+        gp0 = gsum<K>(gp0);
+        gp1 = gsum<K>(gp1);
+    }
+    if (P.coulomb) dvi = gsum<K>(dvi);
+
+    // Slater ratio R = phi(r_p') . Ainv[:,p] (ref slaterdeterminant.cpp:128-132)
+    R Rs = R(1), Sl = R(0), gsn = R(0);
+    if (P.has_slater) {
+        const R s = w.sw(P);
+        R hx[WK::S + 1], hy[WK::S + 1];
+        hermite<R, WK::S>(s * xn0, hx);
+        hermite<R, WK::S>(s * xn1, hy);
+        R Tl = R(0);
+#pragma unroll
+        for (int kk = 0; kk < NP; kk++) {
+            const int nx = orb_nx(kk), ny = orb_ny(kk);
+            const R c = vsel(slot, w.dn[kk], w.up[kk]);
+            Sl += hx[nx] * hy[ny] * c;
+            if (P.sampling == kImportanceExact) {
+                const R dxp = nx ? R(kHC.d1[nx]) * hx[nx ? nx - 1 : 0] * hy[ny] : R(0);
+                const R dyp = ny ? R(kHC.d1[ny]) * hx[nx] * hy[ny ? ny - 1 : 0] : R(0);
+                Tl += (d ? dyp : dxp) * c;
+            }
+        }
+        Rs = __shfl(Sl, osrc, 64);
+        if (P.sampling == kImportanceExact) gsn = s * __shfl(Tl, osrc, 64) / Rs;
+    }
+
+    // acceptance (ref importancesampling.cpp:43-47, 68-81; bruteforce.cpp:42-52)
+    info.rs2 = Rs * Rs;
+    info.rp = P.has_pade ? M<R>::exp(R(2) * jexp) : R(1);
+    info.rg = P.has_gauss ? M<R>::exp(oa * (xpd * xpd - xnd * xnd)) : R(1);
+    const R ratio = info.rs2 * info.rp * info.rg;
+    bool accept;
+    if (forced) {
+        accept = true;
+    } else if (P.sampling == kBruteForce) {
+        accept = !(ratio < ua);
+    } else if (P.sampling == kImportanceCompat) {
+        const R wgt = (M<R>::exp(R(0.5) * (stale - F) * dx) + R(1)) * ratio;
+        accept = !(wgt < ua);
+    } else {
+        R Gn = R(0);
+        if (P.has_slater) Gn += gsn;
+        if (P.has_pade) Gn += d ? gp1 : gp0;
+        if (P.has_gauss) Gn -= oa * xnd;
+        const R Fr = R(2) * Gn;
+        const R Fn = limit ? R(2) * Fr / (R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * Fr * Fr * dt))
+                           : Fr;
+        const R lg = R(-0.5) * dx * (Fp + Fn) + dt * R(0.125) * (Fp * Fp - Fn * Fn);
+        accept = !(ratio * M<R>::exp(lg) < ua);
+    }
+    if (!forced && !(ratio == ratio)) accept = false;
+    if (!accept) return false;
+
+    const bool owner = (w.l == lp);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const bool me = owner && s == slot;
+        if (me) {
+            if (d) w.x[s][1] = xnd; else w.x[s][0] = xnd;
+            if (P.sampling == kImportanceCompat) {
+                if (d) w.qf[s][1] = F; else w.qf[s][0] = F;
+            }
+        }
+        if (P.has_pade) {
+            w.gJ[s][0] = me ? gp0 : w.gJ[s][0] + (w.valid[s] ? dg[s][0] : R(0));
+            w.gJ[s][1] = me ? gp1 : w.gJ[s][1] + (w.valid[s] ? dg[s][1] : R(0));
+        }
+    }
+    w.sumx2 += double(xnd) * double(xnd) - double(xpd) * double(xpd);
+    // only group-reduced increments may enter the walker sums
+    if (P.coulomb) w.vint += double(dvi);
+    if (P.has_pade) {
+        w.lj += double(dlj);
+        w.jg += double(djg);
+    }
+
+    if (P.has_slater) {
+        // Sherman-Morrison update of the moved block (ref slaterdeterminant.cpp:143-153):
+        // column j -= (S_j/R) column p, column p /= R, written as one FMA per entry
+        const R iR = R(1) / Rs;
+        const R cf = owner ? (R(1) - iR) : Sl * iR;
+        R col[NP];
+#pragma unroll
+        for (int kk = 0; kk < NP; kk++) {
+            const R c = vsel(slot, w.dn[kk], w.up[kk]);
+            const R cp = __shfl(c, osrc, 64);
+            col[kk] = c - cf * cp;
+            w.up[kk] = vsel(slot, w.up[kk], col[kk]);
+            w.dn[kk] = vsel(slot, col[kk], w.dn[kk]);
+        }
+        R gx, gy, sec;
+        w.slater_derivs(P, slot, col, gx, gy, sec);
+        if (w.l < NP) {
+            if (slot) { w.gS[1][0] = gx; w.gS[1][1] = gy; w.sS[1] = sec; }
+            else { w.gS[0][0] = gx; w.gS[0][1] = gy; w.sS[0] = sec; }
+        }
+    }
+    w.local_energy(P);
+    return true;
+}
+
+template <typename R, int NP, int K>
+__device__ __forceinline__ void walker_load(Walker<R, NP, K>& w, const WalkParams& P,
+                                            const double* pos, const double* qf)
+{
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int q = s * NP + w.l;
+        w.valid[s] = (w.l < NP) && (q < P.N);
+        w.upspin[s] = q < P.n_up;
+        w.x[s][0] = w.valid[s] ? R(pos[2 * q]) : R(0);
+        w.x[s][1] = w.valid[s] ? R(pos[2 * q + 1]) : R(0);
+        if (qf && w.valid[s]) {
+            w.qf[s][0] = R(qf[2 * q]);
+            w.qf[s][1] = R(qf[2 * q + 1]);
+        } else {
+            w.qf[s][0] = w.qf[s][1] = R(0);
+        }
+        w.gS[s][0] = w.gS[s][1] = w.sS[s] = R(0);
+    }
+#pragma unroll
+    for (int k = 0; k < NP; k++) w.up[k] = w.dn[k] = R(0);
+}
+
+// full gradient d ln psi / dx of owned particle s, dimension dd
+template <typename R, int NP, int K>
+__device__ __forceinline__ R walker_gradient(const Walker<R, NP, K>& w, const WalkParams& P, int s,
+                                             int dd)
+{
+    R g = R(0);
+    if (P.has_slater) g += w.gS[s][dd];
+    if (P.has_pade) g += w.gJ[s][dd];
+    if (P.has_gauss) g -= R(P.omega * P.alpha) * w.x[s][dd];
+    return g;
+}
+
+// ------------------------------------------------------------------------
+// the walk kernel: equil + steps Metropolis steps per walker
+// ------------------------------------------------------------------------
+template <typename R, int NP, int K>
+__global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
+{
+    using WK = Walker<R, NP, K>;
+    constexpr int G = WK::G;
+    __shared__ double lds_T[G * NP * NP];
+    __shared__ double lds_row[2 * G * NP];
+
+    WK w;
+    const int lane = threadIdx.x;
+    w.l = lane & (K - 1);
+    w.gbase = lane & ~(K - 1);
+    w.g = lane / K;
+    const int walker_raw = blockIdx.x * G + w.g;
+    const bool live = walker_raw < P.W;
+    const int walker = live ? walker_raw : P.W - 1;
+    const int nd = 2 * P.N;
+
+    walker_load(w, P, P.pos + (size_t)walker * nd, P.qf ? P.qf + (size_t)walker * nd : nullptr);
+    if (P.has_slater) w.rebuild_slater(P, lds_T, lds_row);
+    w.rebuild_pairs(P);
+    w.local_energy(P);
+    if (P.init_qf && P.t_begin == 0) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            w.qf[s][0] = R(2) * walker_gradient(w, P, s, 0);
+            w.qf[s][1] = R(2) * walker_gradient(w, P, s, 1);
+        }
+    }
+
+    double aE = 0, aE2 = 0, aT = 0, aV = 0, aI = 0, aT2 = 0, aV2 = 0, aI2 = 0, aAcc = 0;
+    double aGa = 0, aGg = 0, aGaE = 0, aGgE = 0;
+    int bad = 0;
+
+    // random draws for K steps at a time, one step per lane of the group
+    int rk = 0;
+    R rz = R(0), rum = R(0), rua = R(0);
+
+    for (int t = P.t_begin; t < P.t_end; t++) {
+        const int tk = t & (K - 1);
+        if (tk == 0 || t == P.t_begin)
+            step_draws<R>(P.seed, (uint32_t)walker, P.counter0 + (uint64_t)(t - tk + w.l), nd, rk,
+                          rz, rum, rua);
+        const int src = w.gbase + tk;
+        const int k = __shfl(rk, src, 64);
+        const R z = __shfl(rz, src, 64);
+        const R um = __shfl(rum, src, 64);
+        const R ua = __shfl(rua, src, 64);
+
+        StepInfo<R> info;
+        const bool accept = walker_step(w, P, k, z, um, ua, false, R(0), info);
+
+        if (t >= P.equil) {
+            const double E = w.el;
+            if (!(E - E == 0.0)) bad++;
+            aE += E;
+            aE2 += E * E;
+            aT += w.tk;
+            aV += w.vext;
+            aI += w.vint;
+            aT2 += w.tk * w.tk;
+            aV2 += w.vext * w.vext;
+            aI2 += w.vint * w.vint;
+            aAcc += accept ? 1.0 : 0.0;
+            if (t > P.window_lo && t < P.window_hi) {      // ref sampler.cpp:82-86
+                aGa += w.ga;
+                aGg += w.gg;
+                aGaE += w.ga * E;
+                aGgE += w.gg * E;
+            }
+            if (P.rec && live && walker < P.rec_walkers && w.l == 0)
+                P.rec[(size_t)walker * P.steps + (t - P.equil)] = float(E);
+        }
+    }
+
+    if (!live) return;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (!w.valid[s]) continue;
+        const int q = s * NP + w.l;
+        double* dst = P.pos + (size_t)walker * nd + 2 * q;
+        dst[0] = double(w.x[s][0]);
+        dst[1] = double(w.x[s][1]);
+        if (P.qf) {
+            double* qd = P.qf + (size_t)walker * nd + 2 * q;
+            qd[0] = double(w.qf[s][0]);
+            qd[1] = double(w.qf[s][1]);
+        }
+    }
+    if (w.l == 0) {
+        double* a = P.acc + (size_t)walker * kNumWalkerAcc;
+        const double v[kNumWalkerAcc] = {aE, aE2, aT, aV, aI, aT2, aV2, aI2, aAcc,
+                                         aGa, aGg, aGaE, aGgE, double(bad)};
+#pragma unroll
+        for (int i = 0; i < kNumWalkerAcc; i++) a[i] = P.acc_add ? a[i] + v[i] : v[i];
+    }
+}
+
+// ------------------------------------------------------------------------
+// parity kernel: evaluate, force one move through the incremental path,
+// evaluate again (see vmc_evaluate in include/vmc.h)
+// ------------------------------------------------------------------------
+template <typename R, int NP, int K>
+__global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const double* pos,
+                                                  const int* moves, const double* newv,
+                                                  double* out0, double* out1)
+{
+    using WK = Walker<R, NP, K>;
+    constexpr int G = WK::G;
+    __shared__ double lds_T[G * NP * NP];
+    __shared__ double lds_row[2 * G * NP];
+    WK w;
+    const int lane = threadIdx.x;
+    w.l = lane & (K - 1);
+    w.gbase = lane & ~(K - 1);
+    w.g = lane / K;
+    const int walker_raw = blockIdx.x * G + w.g;
+    const bool live = walker_raw < P.W;
+    const int walker = live ? walker_raw : P.W - 1;
+    const int nd = 2 * P.N;
+    const int width = 9 + nd;
+    walker_load(w, P, pos + (size_t)walker * nd, nullptr);
+    if (P.has_slater) w.rebuild_slater(P, lds_T, lds_row);
+    w.rebuild_pairs(P);
+    w.local_energy(P);
+    for (int pass = 0; pass < 2; pass++) {
+        StepInfo<R> info{R(0), R(0), R(0)};
+        if (pass == 1) {
+            walker_step(w, P, moves[walker], R(0), R(0), R(0), true, R(newv[walker]), info);
+        }
+        double* o = (pass ? out1 : out0) + (size_t)walker * width;
+        if (live && w.l == 0) {
+            o[0] = w.el;
+            o[1] = w.tk;
+            o[2] = w.vext;
+            o[3] = w.vint;
+            o[4] = w.ga;
+            o[5] = w.gg;
+            o[6] = double(info.rs2);
+            o[7] = double(info.rp);
+            o[8] = double(info.rg);
+        }
+        if (live) {
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                if (!w.valid[s]) continue;
+                const int q = s * NP + w.l;
+                o[9 + 2 * q] = 2.0 * double(walker_gradient(w, P, s, 0));
+                o[9 + 2 * q + 1] = 2.0 * double(walker_gradient(w, P, s, 1));
+            }
+        }
+    }
+}
+
+#ifndef VMC_WALK_PRECISION
+#define VMC_WALK_PRECISION 0
+#endif
+
+#if VMC_WALK_PRECISION == 32 || VMC_WALK_PRECISION == 64
+// ------------------------------------------------------------------------
+// launch tables of one precision (this translation unit is compiled once per
+// precision so the two halves build in parallel)
+// ------------------------------------------------------------------------
+using RT = std::conditional_t<VMC_WALK_PRECISION == 32, float, double>;
+
+template <int NP, int K>
+static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
+{
+    constexpr int G = 64 / K;
+    const int blocks = (P.W + G - 1) / G;
+    hipLaunchKernelGGL((walk_kernel<RT, NP, K>), dim3(blocks), dim3(64), 0, stream, P);
+    return hipGetLastError();
+}
+
+template <int NP, int K>
+static hipError_t launch_eval_t(const WalkParams& P, const double* pos, const int* moves,
+                                const double* newv, double* o0, double* o1, hipStream_t stream)
+{
+    constexpr int G = 64 / K;
+    const int blocks = (P.W + G - 1) / G;
+    hipLaunchKernelGGL((eval_kernel<RT, NP, K>), dim3(blocks), dim3(64), 0, stream, P, pos, moves,
+                       newv, o0, o1);
+    return hipGetLastError();
+}
+
+#if VMC_WALK_PRECISION == 32
+#define VMC_SUFFIX(name) name##_f32
+#else
+#define VMC_SUFFIX(name) name##_f64
+#endif
+
+hipError_t VMC_SUFFIX(launch_walk)(int np, const WalkParams& P, hipStream_t stream)
+{
+    switch (np) {
+    case 1: return launch_walk_t<1, 1>(P, stream);
+    case 3: return launch_walk_t<3, 4>(P, stream);
+    case 6: return launch_walk_t<6, 8>(P, stream);
+    case 10: return launch_walk_t<10, 16>(P, stream);
+    case 15: return launch_walk_t<15, 16>(P, stream);
+    case 21: return launch_walk_t<21, 32>(P, stream);
+    case 28: return launch_walk_t<28, 32>(P, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t VMC_SUFFIX(launch_eval)(int np, const WalkParams& P, const double* pos,
+                                   const int* moves, const double* newv, double* o0, double* o1,
+                                   hipStream_t stream)
+{
+    switch (np) {
+    case 1: return launch_eval_t<1, 1>(P, pos, moves, newv, o0, o1, stream);
+    case 3: return launch_eval_t<3, 4>(P, pos, moves, newv, o0, o1, stream);
+    case 6: return launch_eval_t<6, 8>(P, pos, moves, newv, o0, o1, stream);
+    case 10: return launch_eval_t<10, 16>(P, pos, moves, newv, o0, o1, stream);
+    case 15: return launch_eval_t<15, 16>(P, pos, moves, newv, o0, o1, stream);
+    case 21: return launch_eval_t<21, 32>(P, pos, moves, newv, o0, o1, stream);
+    case 28: return launch_eval_t<28, 32>(P, pos, moves, newv, o0, o1, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+// the draws of n consecutive steps of one walker, in the walk precision
+template <typename R>
+__global__ void draws_kernel(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
+                             int* k, double* z, double* um, double* ua)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int kk;
+    R zz, m, a;
+    step_draws<R>(seed, walker, counter0 + (uint64_t)i, nd, kk, zz, m, a);
+    k[i] = kk;
+    z[i] = double(zz);
+    um[i] = double(m);
+    ua[i] = double(a);
+}
+
+hipError_t VMC_SUFFIX(launch_draws)(uint64_t seed, uint32_t walker, uint64_t counter0, int nd,
+                                    int n, int* k, double* z, double* um, double* ua,
+                                    hipStream_t stream)
+{
+    hipLaunchKernelGGL(draws_kernel<RT>, dim3((n + 255) / 256), dim3(256), 0, stream, seed, walker,
+                       counter0, nd, n, k, z, um, ua);
+    return hipGetLastError();
+}
+#endif
+
+#if VMC_WALK_PRECISION == 0
+// ------------------------------------------------------------------------
+// deterministic reduction of the per-walker accumulators: one block, fixed order
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void reduce_acc_kernel(const double* acc, int W, double* out)
+{
+    __shared__ double part[256];
+    for (int a = 0; a < kNumWalkerAcc; a++) {
+        double s = 0;
+        for (int i = threadIdx.x; i < W; i += 256) s += acc[(size_t)i * kNumWalkerAcc + a];
+        part[threadIdx.x] = s;
+        __syncthreads();
+        for (int m = 128; m > 0; m >>= 1) {
+            if ((int)threadIdx.x < m) part[threadIdx.x] += part[threadIdx.x + m];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[a] = part[0];
+        __syncthreads();
+    }
+}
+
+// initial positions x ~ N(0, (1/omega)^2) (ref InitialStates/randomnormal.cpp:17)
+__global__ void init_positions_kernel(double* pos, int W, int nd, double omega, uint64_t seed)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blocks = (nd + 1) / 2;
+    if (i >= W * blocks) return;
+    const int w = i / blocks, blk = i % blocks;
+    uint32_t c[4] = {(uint32_t)blk, 0u, (uint32_t)w, 1u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double r = ::sqrt(-2.0 * ::log(u23<double>(c[0])));
+    const double th = 6.283185307179586476925286766559 * u23<double>(c[1]);
+    pos[(size_t)w * nd + 2 * blk] = r * ::cos(th) / omega;
+    if (2 * blk + 1 < nd) pos[(size_t)w * nd + 2 * blk + 1] = r * ::sin(th) / omega;
+}
+
+hipError_t launch_walk_f32(int np, const WalkParams& P, hipStream_t stream);
+hipError_t launch_walk_f64(int np, const WalkParams& P, hipStream_t stream);
+hipError_t launch_eval_f32(int np, const WalkParams& P, const double* pos, const int* moves,
+                           const double* newv, double* o0, double* o1, hipStream_t stream);
+hipError_t launch_eval_f64(int np, const WalkParams& P, const double* pos, const int* moves,
+                           const double* newv, double* o0, double* o1, hipStream_t stream);
+
+hipError_t launch_draws_f32(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
+                            int* k, double* z, double* um, double* ua, hipStream_t stream);
+hipError_t launch_draws_f64(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
+                            int* k, double* z, double* um, double* ua, hipStream_t stream);
+
+hipError_t launch_walk(int precision, int np, const WalkParams& P, hipStream_t stream)
+{
+    return precision == 32 ? launch_walk_f32(np, P, stream) : launch_walk_f64(np, P, stream);
+}
+
+hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int nd,
+                        int n, int* k, double* z, double* um, double* ua, hipStream_t stream)
+{
+    return precision == 32 ? launch_draws_f32(seed, walker, counter0, nd, n, k, z, um, ua, stream)
+                           : launch_draws_f64(seed, walker, counter0, nd, n, k, z, um, ua, stream);
+}
+
+hipError_t launch_eval(int precision, int np, const WalkParams& P, const double* pos,
+                       const int* moves, const double* newv, double* o0, double* o1,
+                       hipStream_t stream)
+{
+    return precision == 32 ? launch_eval_f32(np, P, pos, moves, newv, o0, o1, stream)
+                           : launch_eval_f64(np, P, pos, moves, newv, o0, o1, stream);
+}
+
+int walkers_per_wave(int np)
+{
+    switch (np) {
+    case 1: return 64;
+    case 3: return 16;
+    case 6: return 8;
+    case 10: case 15: return 4;
+    case 21: case 28: return 2;
+    }
+    return 0;
+}
+
+hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(reduce_acc_kernel, dim3(1), dim3(256), 0, stream, acc, W, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
+                                 hipStream_t stream)
+{
+    const int n = W * ((nd + 1) / 2);
+    hipLaunchKernelGGL(init_positions_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, pos, W,
+                       nd, omega, seed);
+    return hipGetLastError();
+}
+#endif
+
+}  // namespace vmc

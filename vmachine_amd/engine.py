@@ -1,0 +1,235 @@
+"""ctypes binding of the C ABI in include/vmc.h (libvmc_hip.so).
+
+The engine runs only through the HIP library: if libvmc_hip.so is missing,
+loading fails with an error that says how to build it; there is no CPU path.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "_lib", "libvmc_hip.so")
+HEADER = os.path.join(ROOT, "include", "vmc.h")
+
+GAUSSIAN, SLATER, PADE, RBM_GAUSSIAN, RBM_PRODUCT = 1, 2, 3, 4, 5
+BRUTE_FORCE, IMPORTANCE, IMPORTANCE_EXACT = 0, 1, 2
+Q2_PADE_RATIO_UPPER, Q3_PADE_GAMMA_HALF = 1 << 2, 1 << 3
+QUIRKS_REFERENCE = Q2_PADE_RATIO_UPPER | Q3_PADE_GAMMA_HALF
+EVAL_E, EVAL_T, EVAL_VEXT, EVAL_VINT, EVAL_GALPHA, EVAL_GGAMMA = range(6)
+EVAL_RATIO_SLATER, EVAL_RATIO_PADE, EVAL_RATIO_GAUSS, EVAL_DRIFT = 6, 7, 8, 9
+
+_D, _I, _U = ctypes.c_double, ctypes.c_int, ctypes.c_uint
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
+
+
+class VmcConfig(ctypes.Structure):
+    _fields_ = [
+        ("num_particles", _I), ("num_dimensions", _I), ("omega", _D), ("interaction", _I),
+        ("num_elements", _I), ("kinds", _I * 6), ("num_hidden", _I), ("sigma", _D),
+        ("sorting", _I), ("sampling", _I), ("step_length", _D), ("quirks", _U),
+        ("precision", _I), ("refresh_interval", _I), ("num_walkers", _I), ("drift_limit", _D),
+    ]
+
+
+class VmcSums(ctypes.Structure):
+    _fields_ = [(n, _D) for n in [
+        "energy", "energy2", "kinetic", "external", "interaction", "kinetic2", "external2",
+        "interaction2", "accepted", "samples", "gradient_samples", "kernel_ms", "nonfinite"]]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class VmcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def header_functions():
+    """Names of the functions include/vmc.h declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(vmc_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VmcError(f"{LIB_PATH} is not built: run `python -m vmachine_amd.build` "
+                           "(hipcc --offload-arch=gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.vmc_last_error.restype = ctypes.c_char_p
+        L.vmc_num_parameters.argtypes = [ctypes.POINTER(VmcConfig), _I]
+        L.vmc_max_parameters.argtypes = [ctypes.POINTER(VmcConfig)]
+        L.vmc_device_count.restype = _I
+        L.vmc_engine_create.argtypes = [ctypes.POINTER(VmcConfig), ctypes.POINTER(ctypes.c_void_p)]
+        L.vmc_engine_destroy.argtypes = [ctypes.c_void_p]
+        L.vmc_engine_destroy.restype = None
+        L.vmc_set_parameters.argtypes = [ctypes.c_void_p, _DP, _I, _I]
+        L.vmc_init_walkers.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        for n in ["vmc_set_positions", "vmc_get_positions", "vmc_set_quantum_force",
+                  "vmc_get_quantum_force"]:
+            getattr(L, n).argtypes = [ctypes.c_void_p, _DP, _I, _I]
+        L.vmc_set_random_state.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+        L.vmc_get_random_state.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.POINTER(ctypes.c_uint64)]
+        L.vmc_run.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, ctypes.POINTER(VmcSums), _DP, _DP]
+        L.vmc_run_recorded.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, ctypes.POINTER(VmcSums),
+                                       _DP, _DP, _I, _FP]
+        L.vmc_evaluate.argtypes = [ctypes.c_void_p, _DP, _I, _IP, _DP, _DP, _DP]
+        L.vmc_draws.argtypes = [_I, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _I, _I,
+                                _IP, _DP, _DP, _DP]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def make_config(num_particles, omega=1.0, interaction=True, kinds=(SLATER, GAUSSIAN, PADE),
+                sampling=IMPORTANCE, step_length=0.01, quirks=QUIRKS_REFERENCE, precision=64,
+                refresh_interval=0, num_walkers=1024, num_hidden=None, sigma=None,
+                sorting=False, num_dimensions=2, drift_limit=0.0):
+    c = VmcConfig()
+    c.drift_limit = drift_limit
+    c.num_particles = num_particles
+    c.num_dimensions = num_dimensions
+    c.omega = omega
+    c.interaction = int(bool(interaction))
+    c.num_elements = len(kinds)
+    for i, k in enumerate(kinds):
+        c.kinds[i] = k
+    c.num_hidden = num_particles if num_hidden is None else num_hidden
+    c.sigma = (1.0 / np.sqrt(omega)) if sigma is None else sigma
+    c.sorting = int(sorting)
+    c.sampling = sampling
+    c.step_length = step_length
+    c.quirks = quirks
+    c.precision = precision
+    c.refresh_interval = refresh_interval
+    c.num_walkers = num_walkers
+    return c
+
+
+def max_parameters(cfg):
+    return lib().vmc_max_parameters(ctypes.byref(cfg))
+
+
+def device_count():
+    return lib().vmc_device_count()
+
+
+def draws(precision, seed, walker, counter0, n, dof):
+    """The walk's random draws (coordinate, normal, move uniform, accept uniform)."""
+    k = np.zeros(n, dtype=np.int32)
+    z, um, ua = np.zeros(n), np.zeros(n), np.zeros(n)
+    L = lib()
+    rc = L.vmc_draws(precision, seed, walker, counter0, n, dof, k.ctypes.data_as(_IP), _dp(z),
+                     _dp(um), _dp(ua))
+    if rc != 0:
+        raise VmcError(f"vmc error {rc}: {L.vmc_last_error().decode()}")
+    return k, z, um, ua
+
+
+class Engine:
+    """One device engine: W walkers of one system (include/vmc.h)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = lib()
+        h = ctypes.c_void_p()
+        self._check(self.L.vmc_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.nd = cfg.num_particles * cfg.num_dimensions
+        self.W = cfg.num_walkers
+        self.maxpar = max_parameters(cfg)
+        self.E = cfg.num_elements
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VmcError(f"vmc error {rc}: {self.L.vmc_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vmc_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_parameters(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(self.E, self.maxpar)
+        self._check(self.L.vmc_set_parameters(self.h, _dp(p), self.E, self.maxpar))
+
+    def init_walkers(self, seed):
+        self._check(self.L.vmc_init_walkers(self.h, seed))
+
+    def set_positions(self, pos):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.W, self.nd)
+        self._check(self.L.vmc_set_positions(self.h, _dp(p), self.W, self.nd))
+
+    def positions(self):
+        out = np.zeros((self.W, self.nd))
+        self._check(self.L.vmc_get_positions(self.h, _dp(out), self.W, self.nd))
+        return out
+
+    def set_quantum_force(self, qf):
+        q = np.ascontiguousarray(qf, dtype=np.float64).reshape(self.W, self.nd)
+        self._check(self.L.vmc_set_quantum_force(self.h, _dp(q), self.W, self.nd))
+
+    def quantum_force(self):
+        out = np.zeros((self.W, self.nd))
+        self._check(self.L.vmc_get_quantum_force(self.h, _dp(out), self.W, self.nd))
+        return out
+
+    def set_random_state(self, seed, counter=0):
+        self._check(self.L.vmc_set_random_state(self.h, seed, counter))
+
+    def random_state(self):
+        s, c = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.vmc_get_random_state(self.h, ctypes.byref(s), ctypes.byref(c)))
+        return s.value, c.value
+
+    def run(self, steps, equilibration=0, iteration=0, batches=1, record_walkers=0):
+        sums = VmcSums()
+        g = np.zeros(max(self.E * self.maxpar, 1))
+        ge = np.zeros_like(g)
+        if record_walkers:
+            en = np.zeros((record_walkers, steps), dtype=np.float32)
+            self._check(self.L.vmc_run_recorded(self.h, steps, equilibration, iteration, batches,
+                                                ctypes.byref(sums), _dp(g), _dp(ge),
+                                                record_walkers,
+                                                en.ctypes.data_as(_FP)))
+        else:
+            en = None
+            self._check(self.L.vmc_run(self.h, steps, equilibration, iteration, batches,
+                                       ctypes.byref(sums), _dp(g), _dp(ge)))
+        n = self.E * self.maxpar
+        return (sums.as_dict(), g[:n].reshape(self.E, self.maxpar),
+                ge[:n].reshape(self.E, self.maxpar), en)
+
+    def evaluate(self, positions, moves, new_values):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        W = pos.shape[0]
+        mv = np.ascontiguousarray(moves, dtype=np.int32)
+        nv = np.ascontiguousarray(new_values, dtype=np.float64)
+        width = 9 + self.nd
+        o0 = np.zeros((W, width))
+        o1 = np.zeros((W, width))
+        self._check(self.L.vmc_evaluate(self.h, _dp(pos), W, mv.ctypes.data_as(_IP), _dp(nv),
+                                        _dp(o0), _dp(o1)))
+        return o0, o1
