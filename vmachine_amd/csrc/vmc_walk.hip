@@ -154,11 +154,39 @@ __device__ __forceinline__ void step_draws(uint64_t seed, uint32_t walker, uint6
 // ------------------------------------------------------------------------
 // K-lane group collectives
 // ------------------------------------------------------------------------
+// data-parallel-primitive lane permutations (one VALU op, no LDS crossbar):
+// quad_perm(1,0,3,2), quad_perm(2,3,0,1), row_half_mirror, row_mirror
+enum : int { kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140 };
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL,
+                                                              0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// sum over the K lanes of a group, every lane receives it.  Each step adds the
+// value of a partner lane, and partners pair up symmetrically, so all lanes
+// form the same floating-point sums.  Steps 1, 2, 4, 8 use DPP (after the
+// quad steps, the half mirror pairs the two quads of 8 lanes and the mirror
+// the two halves of 16); 16 and 32 cross DPP rows and use a lane shuffle.
 template <int K, typename T>
 __device__ __forceinline__ T gsum(T v)
 {
-#pragma unroll
-    for (int m = K / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if constexpr (K >= 2) v += dpp<kDppXor1>(v);
+    if constexpr (K >= 4) v += dpp<kDppXor2>(v);
+    if constexpr (K >= 8) v += dpp<kDppHalfMirror>(v);
+    if constexpr (K >= 16) v += dpp<kDppMirror>(v);
+    if constexpr (K >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (K >= 64) v += __shfl_xor(v, 32, 64);
     return v;
 }
 
@@ -200,7 +228,12 @@ struct Walker {
 
     // owned particles: slot 0 = particle l, slot 1 = particle NP + l
     R x[2][2];
-    R up[NP], dn[NP];      // column l of the spin-up / spin-down Slater inverse
+    // column l of the two Slater inverses: a0 holds the block of spin `act`
+    // (0 up, 1 down), a1 the other one.  A step brings the moved block into a0
+    // first (a register swap, needed on about half the steps), so the ratio,
+    // the Sherman-Morrison update and the derivative refresh address a0 only.
+    R a0[NP], a1[NP];
+    int act;
     R gS[2][2];            // d ln det / dx of the owned particles
     R sS[2];               // sum_d d^2 det / dx_d^2 / det of the owned particles
     R gJ[2][2];            // d ln J / dx of the owned particles (Pade)
@@ -213,12 +246,13 @@ struct Walker {
 
     __device__ __forceinline__ R sw(const WalkParams& P) const { return R(::sqrt(P.omega)); }
 
-    // Slater log-derivatives of particle (b*NP + l) from column `col` of its block inverse
-    __device__ __forceinline__ void slater_derivs(const WalkParams& P, int b, const R (&col)[NP],
-                                                  R& gx, R& gy, R& sec) const
+    // Slater log-derivatives of a particle at (px, py) from its column `col` of
+    // the block inverse
+    __device__ __forceinline__ void slater_derivs(const WalkParams& P, R px, R py,
+                                                  const R (&col)[NP], R& gx, R& gy, R& sec) const
     {
         const R s = sw(P);
-        const R t0 = s * x[b][0], t1 = s * x[b][1];
+        const R t0 = s * px, t1 = s * py;
         R hx[S + 1], hy[S + 1];
         hermite<R, S>(t0, hx);
         hermite<R, S>(t1, hy);
@@ -307,12 +341,13 @@ struct Walker {
 
     __device__ __forceinline__ void rebuild_slater(const WalkParams& P, double* lds_T, double* lds_row)
     {
-        invert_block(P, 0, lds_T, lds_row, up);
-        invert_block(P, 1, lds_T, lds_row, dn);
+        invert_block(P, 0, lds_T, lds_row, a0);
+        invert_block(P, 1, lds_T, lds_row, a1);
+        act = 0;
         R gx, gy, sec;
-        slater_derivs(P, 0, up, gx, gy, sec);
+        slater_derivs(P, x[0][0], x[0][1], a0, gx, gy, sec);
         gS[0][0] = gx; gS[0][1] = gy; sS[0] = sec;
-        slater_derivs(P, 1, dn, gx, gy, sec);
+        slater_derivs(P, x[1][0], x[1][1], a1, gx, gy, sec);
         gS[1][0] = gx; gS[1][1] = gy; sS[1] = sec;
         if (l >= NP) {
             gS[0][0] = gS[0][1] = gS[1][0] = gS[1][1] = R(0);
@@ -450,7 +485,8 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
     // drift of the proposal: the reference's F, or in exact Langevin with a drift
     // limit a > 0, 2F / (1 + sqrt(1 + a F^2 dt / 2))
     const bool limit = P.sampling == kImportanceExact && P.drift_a > 0;
-    const R Fp = limit ? R(2) * F / (R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * F * F * dt)) : F;
+    const R Fp =
+        limit ? R(2) * F * M<R>::rcp(R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * F * F * dt)) : F;
     R dx;
     if (forced) dx = xforced - xpd;
     else if (P.sampling == kBruteForce) dx = R(10) * (um - R(0.5)) * dt;
@@ -496,26 +532,48 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
     if (P.coulomb) dvi = gsum<K>(dvi);
 
     // Slater ratio R = phi(r_p') . Ainv[:,p] (ref slaterdeterminant.cpp:128-132)
-    R Rs = R(1), Sl = R(0), gsn = R(0);
+    R Rs = R(1), Sl = R(0), gsn = R(0), iR = R(1);
     if (P.has_slater) {
+        // bring the moved spin block into a0 (wave-uniform branch: skipped when
+        // no walker of the wave needs it)
+        const bool need = slot != w.act;
+        if (__any(need)) {
+#pragma unroll
+            for (int kk = 0; kk < NP; kk++) {
+                const R t0 = w.a0[kk], t1 = w.a1[kk];
+                w.a0[kk] = need ? t1 : t0;
+                w.a1[kk] = need ? t0 : t1;
+            }
+            w.act = slot;
+        }
         const R s = w.sw(P);
         R hx[WK::S + 1], hy[WK::S + 1];
         hermite<R, WK::S>(s * xn0, hx);
         hermite<R, WK::S>(s * xn1, hy);
-        R Tl = R(0);
+        // phi(r').col = sum_nx h_nx(x') u[nx], u[nx] = sum_ny h_ny(y') col(nx, ny); the
+        // derivative rows reuse u (d/dx') and v[ny] = sum_nx h_nx(x') col (d/dy')
+        const bool exact = P.sampling == kImportanceExact;
+        R u[WK::S + 1], v[WK::S + 1];
+#pragma unroll
+        for (int i = 0; i <= WK::S; i++) u[i] = v[i] = R(0);
 #pragma unroll
         for (int kk = 0; kk < NP; kk++) {
             const int nx = orb_nx(kk), ny = orb_ny(kk);
-            const R c = vsel(slot, w.dn[kk], w.up[kk]);
-            Sl += hx[nx] * hy[ny] * c;
-            if (P.sampling == kImportanceExact) {
-                const R dxp = nx ? R(kHC.d1[nx]) * hx[nx ? nx - 1 : 0] * hy[ny] : R(0);
-                const R dyp = ny ? R(kHC.d1[ny]) * hx[nx] * hy[ny ? ny - 1 : 0] : R(0);
-                Tl += (d ? dyp : dxp) * c;
+            u[nx] += hy[ny] * w.a0[kk];
+            if (exact) v[ny] += hx[nx] * w.a0[kk];
+        }
+        R Tx = R(0), Ty = R(0);
+#pragma unroll
+        for (int i = 0; i <= WK::S; i++) {
+            Sl += hx[i] * u[i];
+            if (exact && i >= 1) {
+                Tx += R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0] * u[i];
+                Ty += R(kHC.d1[i]) * hy[i >= 1 ? i - 1 : 0] * v[i];
             }
         }
         Rs = __shfl(Sl, osrc, 64);
-        if (P.sampling == kImportanceExact) gsn = s * __shfl(Tl, osrc, 64) / Rs;
+        iR = M<R>::rcp(Rs);
+        if (exact) gsn = s * __shfl(d ? Ty : Tx, osrc, 64) * iR;
     }
 
     // acceptance (ref importancesampling.cpp:43-47, 68-81; bruteforce.cpp:42-52)
@@ -537,8 +595,9 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
         if (P.has_pade) Gn += d ? gp1 : gp0;
         if (P.has_gauss) Gn -= oa * xnd;
         const R Fr = R(2) * Gn;
-        const R Fn = limit ? R(2) * Fr / (R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * Fr * Fr * dt))
-                           : Fr;
+        const R Fn =
+            limit ? R(2) * Fr * M<R>::rcp(R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * Fr * Fr * dt))
+                  : Fr;
         const R lg = R(-0.5) * dx * (Fp + Fn) + dt * R(0.125) * (Fp * Fp - Fn * Fn);
         accept = !(ratio * M<R>::exp(lg) < ua);
     }
@@ -571,19 +630,15 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
     if (P.has_slater) {
         // Sherman-Morrison update of the moved block (ref slaterdeterminant.cpp:143-153):
         // column j -= (S_j/R) column p, column p /= R, written as one FMA per entry
-        const R iR = R(1) / Rs;
         const R cf = owner ? (R(1) - iR) : Sl * iR;
-        R col[NP];
 #pragma unroll
         for (int kk = 0; kk < NP; kk++) {
-            const R c = vsel(slot, w.dn[kk], w.up[kk]);
-            const R cp = __shfl(c, osrc, 64);
-            col[kk] = c - cf * cp;
-            w.up[kk] = vsel(slot, w.up[kk], col[kk]);
-            w.dn[kk] = vsel(slot, col[kk], w.dn[kk]);
+            const R cp = __shfl(w.a0[kk], osrc, 64);
+            w.a0[kk] -= cf * cp;
         }
         R gx, gy, sec;
-        w.slater_derivs(P, slot, col, gx, gy, sec);
+        w.slater_derivs(P, vsel(slot, w.x[1][0], w.x[0][0]), vsel(slot, w.x[1][1], w.x[0][1]),
+                        w.a0, gx, gy, sec);
         if (w.l < NP) {
             if (slot) { w.gS[1][0] = gx; w.gS[1][1] = gy; w.sS[1] = sec; }
             else { w.gS[0][0] = gx; w.gS[0][1] = gy; w.sS[0] = sec; }
@@ -613,7 +668,8 @@ __device__ __forceinline__ void walker_load(Walker<R, NP, K>& w, const WalkParam
         w.gS[s][0] = w.gS[s][1] = w.sS[s] = R(0);
     }
 #pragma unroll
-    for (int k = 0; k < NP; k++) w.up[k] = w.dn[k] = R(0);
+    for (int k = 0; k < NP; k++) w.a0[k] = w.a1[k] = R(0);
+    w.act = 0;
 }
 
 // full gradient d ln psi / dx of owned particle s, dimension dd
