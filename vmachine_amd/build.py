@@ -1,4 +1,5 @@
-"""Builds libvmc_hip.so in-tree (vmachine_amd/_lib/) with hipcc for gfx950.
+"""Builds libvmc_hip.so in-tree (vmachine_amd/_lib/) with hipcc for gfx950, and
+the `vmachine` executable (host C++ over the library's C ABI) beside it.
 
 Usage: python -m vmachine_amd.build   (or __graft_entry__.build())
 """
@@ -14,6 +15,10 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIBDIR, "libvmc_hip.so")
 OBJDIR = os.path.join(PKG, "_lib", "obj")
+EXE = os.path.join(LIBDIR, "vmachine")
+HOST_LIB = os.path.join(LIBDIR, "libvmc_host.so")
+HOST_SOURCES = ["config.cpp", "runner.cpp", "cli.cpp"]
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
 
 ARCH = os.environ.get("VMC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -52,9 +57,33 @@ def build(verbose=True):
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_exe()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", EXE)
     return LIB
+
+
+def _host(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed:\n" + r.stdout)
+
+
+def build_exe():
+    """The host layer is plain C++17 (g++): libvmc_host.so reaches the GPU only
+    through libvmc_hip.so's C ABI, and `vmachine` is a one-line main over it."""
+    inc = "-I" + os.path.join(ROOT, "include")
+    srcs = [os.path.join(CSRC, "host", f) for f in HOST_SOURCES]
+    tmp = HOST_LIB + ".tmp"
+    _host([CXX, "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", inc, "-o", tmp] + srcs
+          + ["-L" + LIBDIR, "-lvmc_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    tmp = EXE + ".tmp"
+    _host([CXX, "-O2", "-std=c++17", "-Wall", inc, "-o", tmp,
+           os.path.join(CSRC, "host", "main.cpp"), "-L" + LIBDIR, "-lvmc_host", "-lvmc_hip",
+           "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, EXE)
+    return EXE
 
 
 if __name__ == "__main__":
