@@ -40,7 +40,11 @@ extern "C" {
 /* ---- reference behaviours that change results (set bit = reproduce) ---- */
 #define VMC_Q2_PADE_RATIO_UPPER (1u << 2)  /* ref padejastrow.cpp:214 */
 #define VMC_Q3_PADE_GAMMA_HALF (1u << 3)   /* ref padejastrow.cpp:114 */
-#define VMC_QUIRKS_REFERENCE (VMC_Q2_PADE_RATIO_UPPER | VMC_Q3_PADE_GAMMA_HALF)
+#define VMC_Q6_RBM_W_UNSCALED (1u << 6)   /* ref rbmproduct.cpp:77-83: dlnpsi/dW = x n^T, no 1/sigma_p^2 */
+#define VMC_Q13_RBMG_PSI_RATIO (1u << 13) /* ref rbmgaussian.cpp:44-45: ratio psi'/psi, not |psi'/psi|^2 */
+#define VMC_QUIRKS_REFERENCE                                                   \
+    (VMC_Q2_PADE_RATIO_UPPER | VMC_Q3_PADE_GAMMA_HALF | VMC_Q6_RBM_W_UNSCALED | \
+     VMC_Q13_RBMG_PSI_RATIO)
 
 typedef struct vmc_config {
     int num_particles;      /* numParticles */
@@ -158,6 +162,17 @@ int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration,
 #define VMC_EVAL_WIDTH(dof) (9 + (dof))
 int vmc_evaluate(vmc_engine* e, const double* positions, int walkers, const int* moves,
                  const double* new_values, double* out_before, double* out_after);
+
+/* As vmc_evaluate, and per element (any kind, including the RBM elements):
+ * ratios [walkers][num_elements] = the element's evaluateRatio of the forced
+ * move, grads_before / grads_after [walkers][num_elements x max parameters] =
+ * the parameter-gradient matrix (row e = element e's computeParameterGradient)
+ * before and after it.  Replaces WaveFunction::evaluateRatio and
+ * computeParameterGradient of each element (ref src/WaveFunctions/wavefunction.h:29-32,
+ * src/elementoperation.cpp:80-87, 111-123). */
+int vmc_evaluate_elements(vmc_engine* e, const double* positions, int walkers, const int* moves,
+                          const double* new_values, double* out_before, double* out_after,
+                          double* ratios, double* grads_before, double* grads_after);
 
 /* Parity entry point for the random-number protocol: the draws of steps
  * counter0 .. counter0+n-1 of one walker exactly as the walk computes them in

@@ -41,7 +41,7 @@ enum {
     VMCO_Q2_PADE_RATIO_UPPER = 1u << 2,  /* padejastrow.cpp:214: only pairs j > i_p in the ratio */
     VMCO_Q3_PADE_GAMMA_HALF = 1u << 3,   /* padejastrow.cpp:114: extra 1/2 in dlnpsi/dgamma     */
     VMCO_Q6_RBM_W_NO_SIGMA = 1u << 6,    /* rbmproduct.cpp:98: dlnpsi/dW lacks 1/sigma_p^2       */
-    VMCO_Q13_RBMG_SQRT_RATIO = 1u << 13, /* rbmgaussian.cpp:262-263: ratio uses 2 sigma^2        */
+    VMCO_Q13_RBMG_SQRT_RATIO = 1u << 13, /* rbmgaussian.cpp:44-45: ratio uses 2 sigma^2            */
     VMCO_QUIRKS_ALL = (1u << 2) | (1u << 3) | (1u << 6) | (1u << 13)
 };
 

@@ -76,3 +76,59 @@ def oracle_evaluate(ocfg, P, positions, moves, new_values):
         if O.GAUSSIAN not in kinds:
             out1[i, 8] = 1.0
     return out0, out1
+
+
+def oracle_evaluate_elements(ocfg, P, positions, moves, new_values):
+    """The oracle's version of vmc_evaluate_elements: (o0, o1) as oracle_evaluate,
+    per-element ratios of the forced move, and parameter-gradient matrices."""
+    kinds = [ocfg.kinds[i] for i in range(ocfg.num_elements)]
+    W, nd = positions.shape
+    E = len(kinds)
+    mp = O.max_parameters(ocfg)
+    o0, o1 = np.zeros((W, 9 + nd)), np.zeros((W, 9 + nd))
+    r = np.zeros((W, E))
+    g0, g1 = np.zeros((W, E, mp)), np.zeros((W, E, mp))
+
+    def fill(w, row, g):
+        T, V, I = w.kinetic(), w.external(), w.interaction()
+        row[0:4] = [T + V + I, T, V, I]
+        for e, k in enumerate(kinds):
+            pg = w.parameter_gradient(e)
+            g[e, :] = pg
+            if k == O.GAUSSIAN:
+                row[4] = pg[0]
+            if k == O.PADE:
+                row[5] = pg[0]
+        row[9:] = w.drift()
+
+    for i in range(W):
+        w = O.Walker(ocfg, P, positions[i])
+        fill(w, o0[i], g0[i])
+        w.propose(int(moves[i]), float(new_values[i]))
+        for e in range(E):
+            r[i, e] = w.ratio(e)
+        w.accept()
+        fill(w, o1[i], g1[i])
+    return o0, o1, r, g0, g1
+
+
+def rbm_parameters(rng, kinds, nd, H):
+    """A parameter matrix with Xavier-like RBM weights (ref xavier.cpp:10-23)."""
+    mp = 1
+    for k in kinds:
+        if k == O.RBM_GAUSSIAN:
+            mp = max(mp, nd)
+        if k == O.RBM_PRODUCT:
+            mp = max(mp, H + nd * H)
+    P = np.zeros((len(kinds), mp))
+    a = 1.0 / np.sqrt(nd)
+    for e, k in enumerate(kinds):
+        if k == O.GAUSSIAN:
+            P[e, 0] = 1.0 + 0.1 * rng.standard_normal()
+        elif k == O.PADE:
+            P[e, 0] = 0.3 + 0.7 * rng.random()
+        elif k == O.RBM_GAUSSIAN:
+            P[e, :nd] = rng.uniform(-a, a, nd)
+        elif k == O.RBM_PRODUCT:
+            P[e, :H + nd * H] = rng.uniform(-a, a, H + nd * H)
+    return P

@@ -73,6 +73,12 @@ def test_twelve_electron_dot_example():
     _compare("large_fermionic_quantumdot_vmc", rows, final)
 
 
+def test_rbm_example():
+    # RBM-Gaussian x RBM-product, N=2, H=2, Xavier weights, 500 ADAM iterations
+    rows, final, _ = _run("small_fermionic_quantumdot_rbm", "--seed", 17)
+    _compare("small_fermionic_quantumdot_rbm", rows, final)
+
+
 def test_resampling_example_blocks_the_last_iteration():
     rows, final, out = _run("resampling", "--seed", 14)
     _compare("resampling", rows, final)

@@ -1,0 +1,169 @@
+"""RBM-Gaussian and RBM-product elements in the HIP walk kernel against the
+FP64 oracle (whose RBM functions are pinned to the reference's golden vectors
+el_n2_rbm, el_n12_rbmpj_sorted/unsorted and it_n12_rbmpj_adam in
+test_oracle_golden.py).
+
+Per configuration the engine and the oracle evaluate the same positions and
+parameters, force the same move through the incremental path and evaluate
+again: local energy and parts, drift, per-element ratios and the full
+parameter-gradient matrices (600 RBM-product parameters at N=12, H=24).  FP64
+trajectories must retrace the oracle's chains step for step, including the
+gradient-window sums of every RBM parameter.
+
+Tolerances: FP64 1e-9 scaled (as in test_gpu_parity); FP32 2e-3 max / 2e-5
+median on energies and drifts, and for the ratio and parameter gradients
+relative 1e-4 (single-precision pre-activations v = b + W^T x / sigma_p^2 over
+24 terms).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import oracle_evaluate_elements, rbm_parameters
+
+pytestmark = pytest.mark.gpu
+
+RBMPJ = [O.SLATER, O.RBM_GAUSSIAN, O.RBM_PRODUCT, O.PADE]
+RBM = [O.RBM_GAUSSIAN, O.RBM_PRODUCT]
+REF = O.Q2 | O.Q3 | O.Q6 | O.Q13
+
+CASES = [
+    # N, omega, kinds, H, sorting, quirks
+    (2, 1.0, RBM, 2, True, REF),
+    (2, 1.0, RBM, 2, False, REF),
+    (2, 1.0, RBM, 12, True, REF),          # more hidden units than one lane holds
+    (6, 1.0, RBM + [O.PADE], 8, True, REF),
+    (12, 1.0, RBMPJ, 24, True, REF),
+    (12, 1.0, RBMPJ, 24, False, REF),
+    (12, 0.5, RBMPJ, 24, True, 0),          # the exact-form ratios and gradients
+    (2, 1.0, [O.RBM_PRODUCT], 4, True, REF),
+    (6, 1.0, [O.SLATER, O.RBM_GAUSSIAN], 0, True, REF),
+]
+
+
+def _engine(cfg):
+    import vmachine_amd as V
+    if V.device_count() < 1:
+        pytest.fail("no GPU visible: GPU tests must run on the MI355X box")
+    return V.Engine(cfg)
+
+
+def _configs(N, omega, kinds, H, sorting, quirks, sampling, dt, precision, W):
+    import vmachine_amd as V
+    ocfg = O.make_config(N, 2, omega, 1, kinds, H if H else None, None, sorting, sampling, dt,
+                         quirks=quirks, basis_scaled=False)
+    ecfg = V.make_config(N, omega=omega, interaction=True, kinds=kinds, sampling=sampling,
+                         step_length=dt, quirks=quirks, precision=precision, num_walkers=W,
+                         num_hidden=H if H else N, sorting=sorting)
+    return ocfg, ecfg
+
+
+def _scaled(a, b):
+    return np.abs(a - b) / (np.abs(b) + 1.0)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_rbm_evaluate_matches_oracle(case, precision):
+    N, omega, kinds, H, sorting, quirks = CASES[case]
+    rng = np.random.default_rng(300 + case)
+    W, nd = 40, 2 * N
+    ocfg, ecfg = _configs(N, omega, kinds, H, sorting, quirks, O.IMPORTANCE_COMPAT, 0.05,
+                          precision, W)
+    P = rbm_parameters(rng, kinds, nd, H)
+    pos = rng.standard_normal((W, nd)) / np.sqrt(omega)
+    moves = rng.integers(0, nd, W)
+    newv = pos[np.arange(W), moves] + 0.3 * rng.standard_normal(W)
+    print(f"rbm evaluate case={case} N={N} H={H} sorting={sorting} precision={precision}",
+          flush=True)
+    with _engine(ecfg) as eng:
+        eng.set_parameters(P)
+        g0, g1, gr, gg0, gg1 = eng.evaluate_elements(pos, moves, newv)
+    r0, r1, rr, rg0, rg1 = oracle_evaluate_elements(ocfg, P, pos, moves, newv)
+    tol = 1e-9 if precision == 64 else 2e-3
+    tol_g = 1e-9 if precision == 64 else 1e-4
+    for tag, g, r in [("before", g0, r0), ("after", g1, r1)]:
+        drift = r[:, 9:]
+        scale_e = (np.abs(r[:, 1]) + np.abs(r[:, 2]) + np.abs(r[:, 3]) +
+                   0.125 * np.sum(drift ** 2, axis=1) + 1.0)
+        eE = np.abs(g[:, 0] - r[:, 0]) / scale_e
+        eF = np.max(np.abs(g[:, 9:] - drift), axis=1) / (np.max(np.abs(drift), axis=1) + 1.0)
+        print(f"  {tag}: E max {eE.max():.2e} drift max {eF.max():.2e}")
+        assert eE.max() < tol and eF.max() < tol
+        for c in (1, 2, 3):
+            assert np.max(np.abs(g[:, c] - r[:, c]) / scale_e) < tol
+    eR = np.abs(gr - rr) / np.maximum(np.abs(rr), 1.0)
+    eG0, eG1 = _scaled(gg0, rg0), _scaled(gg1, rg1)
+    print(f"  ratios max {eR.max():.2e}  dlnpsi/dtheta max {max(eG0.max(), eG1.max()):.2e}")
+    assert eR.max() < tol_g
+    assert eG0.max() < tol_g and eG1.max() < tol_g
+
+
+TRAJ = [
+    # N, omega, kinds, H, sorting, quirks, sampling, dt, batches, iteration
+    (2, 1.0, RBM, 2, True, REF, O.IMPORTANCE_COMPAT, 0.1, 1, 0),
+    (2, 1.0, RBM, 2, True, REF, O.BRUTE_FORCE, 0.5, 10, 3),
+    (12, 1.0, RBMPJ, 24, True, REF, O.IMPORTANCE_COMPAT, 0.02, 1, 0),
+    (12, 1.0, RBMPJ, 24, False, REF, O.IMPORTANCE_EXACT, 0.02, 10, 7),
+    (6, 1.0, RBM + [O.PADE], 8, True, 0, O.IMPORTANCE_EXACT, 0.05, 1, 0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(TRAJ)))
+def test_rbm_fp64_trajectories_match_oracle(case):
+    """Same Philox streams: accept decisions, positions, energy sums and the
+    gradient-window sums of every parameter agree with the oracle's chains."""
+    N, omega, kinds, H, sorting, quirks, sampling, dt, batches, it = TRAJ[case]
+    rng = np.random.default_rng(500 + case)
+    W, steps, equil, seed = 48, 200, 20, 77 + case
+    nd = 2 * N
+    ocfg, ecfg = _configs(N, omega, kinds, H, sorting, quirks, sampling, dt, 64, W)
+    P = rbm_parameters(rng, kinds, nd, H)
+    pos0 = np.stack([O.initial_positions(seed, w, nd, omega) for w in range(W)]) * np.sqrt(omega)
+    print(f"rbm trajectory case={case} N={N} H={H} sampling={sampling}", flush=True)
+    with _engine(ecfg) as eng:
+        eng.set_parameters(P)
+        eng.set_positions(pos0)
+        eng.set_random_state(seed, 0)
+        sums, g, ge, _ = eng.run(steps, equil, it, batches)
+        gpos = eng.positions()
+    opos = pos0.copy()
+    counters = np.zeros(W, dtype=np.uint64)
+    acc, og, oge = O.run_many(ocfg, P, opos, seed, counters, steps, equil, it, batches, threads=8)
+    og = og.reshape(g.shape)
+    oge = oge.reshape(ge.shape)
+    print(f"  accepted {sums['accepted']:.0f} / {acc[8]:.0f}, E {sums['energy']:.10g} / {acc[0]:.10g}")
+    assert sums["accepted"] == acc[8]
+    assert abs(sums["energy"] - acc[0]) <= 1e-9 * abs(acc[0])
+    assert abs(sums["energy2"] - acc[1]) <= 1e-9 * abs(acc[1])
+    assert np.max(np.abs(gpos - opos)) < 1e-9 * max(1.0, np.max(np.abs(opos)))
+    assert np.max(np.abs(g - og)) <= 1e-9 * max(1.0, np.max(np.abs(og)))
+    assert np.max(np.abs(ge - oge)) <= 1e-9 * max(1.0, np.max(np.abs(oge)))
+    assert np.any(g != 0)
+
+
+def test_rbm_fp32_walk_agrees_with_fp64():
+    """RBMPJ N=12, H=24: FP32 and FP64 walks on the same streams give energies
+    and RBM gradient sums that agree to the FP32 rounding of the chains."""
+    N, omega, kinds, H = 12, 1.0, RBMPJ, 24
+    rng = np.random.default_rng(901)
+    W, nd = 512, 24
+    P = rbm_parameters(rng, kinds, nd, H)
+    res = {}
+    for prec in (64, 32):
+        _, ecfg = _configs(N, omega, kinds, H, True, REF, O.IMPORTANCE_COMPAT, 0.02, prec, W)
+        ecfg.refresh_interval = 100
+        print(f"rbm fp{prec} walk", flush=True)
+        with _engine(ecfg) as eng:
+            eng.set_parameters(P)
+            eng.init_walkers(5)
+            eng.run(1000, 0)
+            sums, g, ge, _ = eng.run(2000, 0, 0, 1)
+        res[prec] = (sums["energy"] / sums["samples"], g / sums["gradient_samples"],
+                     sums["accepted"] / sums["samples"])
+    E64, g64, a64 = res[64]
+    E32, g32, a32 = res[32]
+    print(f"  E {E64:.6f} vs {E32:.6f}, acceptance {a64:.5f} vs {a32:.5f}")
+    assert np.isfinite(E32)
+    assert abs(E32 - E64) < 0.02 * abs(E64)
+    assert abs(a32 - a64) < 0.01

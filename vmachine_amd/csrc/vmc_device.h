@@ -14,6 +14,12 @@ enum : int {
     kNumWalkerAcc
 };
 
+// width of one block's row of RBM gradient-window sums: the RBM-Gaussian row
+// (a_k at k), then the RBM-product row (b_h at h, W[k][h] at H + k + ND*h, the
+// reference's parameter layout, ref rbmproduct.cpp:18-27), for the sums of
+// dlnpsi/dtheta and then of E_L * dlnpsi/dtheta
+constexpr int rbm_row_width(int nd, int h) { return 2 * (nd + h + nd * h); }
+
 struct WalkParams {
     // system
     int N;              // particles
@@ -24,6 +30,19 @@ struct WalkParams {
     double alpha;       // Gaussian alpha
     double gamma;       // Pade gamma
     double a_same, a_anti;
+    // RBM elements (ref WaveFunctions/rbmgaussian.cpp, rbmproduct.cpp)
+    int has_rbmg, has_rbmp;
+    int ND;             // degrees of freedom, 2N
+    int H;              // hidden units
+    int sorting;        // RBM-product sees the sorted coordinate vector
+    double sig2_g;      // RBM-Gaussian sigma^2
+    double isig2_p;     // 1 / sigma_p^2, sigma_p = 10 sigma (ref rbmproduct.cpp:15)
+    int q13_psi_ratio;  // reference RBM-Gaussian ratio exp(d/(2 sigma^2))
+    int q6_w_unscaled;  // reference dlnpsi/dW without 1/sigma_p^2
+    const double* rbm_a;    // [ND]
+    const double* rbm_b;    // [H]
+    const double* rbm_W;    // [ND][H], W[k][h] at k*H + h
+    double* rbm_acc;        // [blocks][rbm_row_width(ND, H)] gradient-window sums per block
     // Metropolis
     int sampling;
     double dt;          // importance dt or brute-force step length
@@ -50,6 +69,24 @@ struct WalkParams {
     float* rec;         // [rec_walkers][steps] local energies, may be null
     int rec_walkers;
     int init_qf;        // 1: set the stored drift to the drift at the loaded positions
+};
+
+// per-walker outputs of the parity kernel
+struct EvalOut {
+    double* o0;        // [W][9 + nd]
+    double* o1;
+    double* ratios;    // [W][num_elements] per-element ratio of the forced move, or null
+    double* g0;        // [W][num_elements * maxpar] dlnpsi/dtheta before, or null (zeroed by the host)
+    double* g1;        // after
+    int rows[6];       // element kind of each parameter row
+    int num_elements, maxpar;
+};
+
+// which kernel instantiation runs a configuration: np particle slots per spin
+// (Slater orbitals per spin), K lanes per walker, RBM elements or not
+struct KernelShape {
+    int np, K;
+    bool rbm;
 };
 
 }  // namespace vmc

@@ -16,16 +16,18 @@
 #include "vmc_device.h"
 
 namespace vmc {
-hipError_t launch_walk(int precision, int np, const WalkParams& P, hipStream_t stream);
-hipError_t launch_eval(int precision, int np, const WalkParams& P, const double* pos,
-                       const int* moves, const double* newv, double* o0, double* o1,
+hipError_t launch_walk(int precision, const KernelShape& sh, const WalkParams& P,
                        hipStream_t stream);
+hipError_t launch_eval(int precision, const KernelShape& sh, const WalkParams& P,
+                       const double* pos, const int* moves, const double* newv, const EvalOut& O,
+                       hipStream_t stream);
+hipError_t launch_reduce_rows(const double* rows, int nrows, int width, double* out,
+                              hipStream_t stream);
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
                                  hipStream_t stream);
 hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int nd,
                         int n, int* k, double* z, double* um, double* ua, hipStream_t stream);
-int walkers_per_wave(int np);
 }  // namespace vmc
 
 namespace {
@@ -46,6 +48,11 @@ int fail(int code, const std::string& msg)
     } while (0)
 
 const int kNP[] = {1, 3, 6, 10, 15, 21, 28};
+const int kLanes[] = {1, 4, 8, 16, 16, 32, 32};   // lanes per walker for each kNP entry
+
+// RBM instantiations: (np, K) pairs compiled with the RBM elements
+const int kRbmShapes[][2] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {3, 4}, {3, 8}, {6, 8}, {10, 16}};
+constexpr int kRbmMaxDof = 32, kRbmMaxHidden = 32;
 
 // closed-shell 2-D states per spin: 2 * C(c+2, 2) == N (ref src/Basis/basis.cpp:106-123)
 int closed_shell_orbitals(int N)
@@ -77,9 +84,16 @@ int element_row(const vmc_config* c, int kind)
 struct vmc_engine {
     vmc_config cfg;
     int np = 0;             // particles per ownership slot (Slater orbitals per spin)
+    vmc::KernelShape shape{};
     int nd = 0;
     int maxpar = 0;
-    int row_gauss = -1, row_pade = -1;
+    int row_gauss = -1, row_pade = -1, row_rbmg = -1, row_rbmp = -1;
+    double* d_rbm_a = nullptr;     // [nd]
+    double* d_rbm_b = nullptr;     // [H]
+    double* d_rbm_W = nullptr;     // [nd][H]
+    double* d_rbm_acc = nullptr;   // [blocks][row width]
+    double* d_rbm_out = nullptr;   // [row width]
+    int rbm_blocks = 0, rbm_width = 0;
     double alpha = 1.0, gamma = 1.0;
     bool params_set = false;
     bool walkers_set = false;
@@ -103,6 +117,8 @@ struct vmc_engine {
         if (d_acc) (void)hipFree(d_acc);
         if (d_out) (void)hipFree(d_out);
         if (d_rec) (void)hipFree(d_rec);
+        for (double* b : {d_rbm_a, d_rbm_b, d_rbm_W, d_rbm_acc, d_rbm_out})
+            if (b) (void)hipFree(b);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -157,9 +173,8 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
         return fail(VMC_E_INVALID, "between 1 and 6 wave-function elements");
     for (int e = 0; e < c.num_elements; e++) {
         const int k = c.kinds[e];
-        if (k == VMC_RBM_GAUSSIAN || k == VMC_RBM_PRODUCT)
-            return fail(VMC_E_INVALID, "RBM elements are not available in the walk kernel yet");
-        if (k != VMC_GAUSSIAN && k != VMC_SLATER && k != VMC_PADE)
+        if (k != VMC_GAUSSIAN && k != VMC_SLATER && k != VMC_PADE && k != VMC_RBM_GAUSSIAN &&
+            k != VMC_RBM_PRODUCT)
             return fail(VMC_E_INVALID, "unknown wave-function element");
         for (int f = 0; f < e; f++)
             if (c.kinds[f] == k) return fail(VMC_E_INVALID, "an element appears twice");
@@ -174,6 +189,14 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
     if (c.refresh_interval < 0) return fail(VMC_E_INVALID, "refresh_interval must be >= 0");
     if (!(c.drift_limit >= 0)) return fail(VMC_E_INVALID, "drift_limit must be >= 0");
 
+    const bool rbm = has(&c, VMC_RBM_GAUSSIAN) || has(&c, VMC_RBM_PRODUCT);
+    if (rbm) {
+        if (2 * c.num_particles > kRbmMaxDof)
+            return fail(VMC_E_INVALID, "RBM elements: numParticles * numDimensions must be <= 32");
+        if (has(&c, VMC_RBM_PRODUCT) && (c.num_hidden < 1 || c.num_hidden > kRbmMaxHidden))
+            return fail(VMC_E_INVALID, "RBM elements: numHiddenNodes must be in 1..32");
+        if (!(c.sigma > 0)) return fail(VMC_E_INVALID, "RBM elements: sigma must be positive");
+    }
     int np;
     if (has(&c, VMC_SLATER)) {
         np = closed_shell_orbitals(c.num_particles);
@@ -193,11 +216,26 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
 
     vmc_engine* e = new vmc_engine();
     e->cfg = c;
+    int K = 0;
+    for (size_t i = 0; i < sizeof(kNP) / sizeof(kNP[0]); i++)
+        if (kNP[i] == np) K = kLanes[i];
+    if (rbm) {
+        // four hidden units per lane: K >= H/4
+        const int need = has(&c, VMC_RBM_PRODUCT) ? (c.num_hidden + 3) / 4 : 1;
+        while (K < need) K *= 2;
+        bool found = false;
+        for (const auto& sh : kRbmShapes) found |= (sh[0] == np && sh[1] == K);
+        if (!found) return fail(VMC_E_INVALID, "no RBM kernel for this particle/hidden-unit count");
+    }
+
     e->np = np;
+    e->shape = vmc::KernelShape{np, K, rbm};
     e->nd = 2 * c.num_particles;
     e->maxpar = vmc_max_parameters(&c);
     e->row_gauss = element_row(&c, VMC_GAUSSIAN);
     e->row_pade = element_row(&c, VMC_PADE);
+    e->row_rbmg = element_row(&c, VMC_RBM_GAUSSIAN);
+    e->row_rbmp = element_row(&c, VMC_RBM_PRODUCT);
     auto cleanup = [&](int code) {
         delete e;
         return code;
@@ -213,6 +251,18 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
         (err = hipMalloc(&e->d_acc, W * vmc::kNumWalkerAcc * sizeof(double))) != hipSuccess ||
         (err = hipMalloc(&e->d_out, vmc::kNumWalkerAcc * sizeof(double))) != hipSuccess)
         return cleanup(fail(VMC_E_DEVICE, std::string("allocation: ") + hipGetErrorString(err)));
+    if (rbm) {
+        const int H = has(&c, VMC_RBM_PRODUCT) ? c.num_hidden : 0, nd = e->nd;
+        e->rbm_blocks = (c.num_walkers + 64 / K - 1) / (64 / K);
+        e->rbm_width = vmc::rbm_row_width(nd, H);
+        if ((err = hipMalloc(&e->d_rbm_a, nd * sizeof(double))) != hipSuccess ||
+            (err = hipMalloc(&e->d_rbm_b, (H + 1) * sizeof(double))) != hipSuccess ||
+            (err = hipMalloc(&e->d_rbm_W, ((size_t)nd * H + 1) * sizeof(double))) != hipSuccess ||
+            (err = hipMalloc(&e->d_rbm_acc, (size_t)e->rbm_blocks * e->rbm_width * sizeof(double))) !=
+                hipSuccess ||
+            (err = hipMalloc(&e->d_rbm_out, (size_t)e->rbm_width * sizeof(double))) != hipSuccess)
+            return cleanup(fail(VMC_E_DEVICE, std::string("allocation: ") + hipGetErrorString(err)));
+    }
     *out = e;
     g_err.clear();
     return VMC_OK;
@@ -227,6 +277,19 @@ int vmc_set_parameters(vmc_engine* e, const double* params, int rows, int cols)
         return fail(VMC_E_INVALID, "parameter matrix must be num_elements x max_parameters");
     if (e->row_gauss >= 0) e->alpha = params[e->row_gauss * cols + 0];
     if (e->row_pade >= 0) e->gamma = params[e->row_pade * cols + 0];
+    const int nd = e->nd, H = e->cfg.num_hidden;
+    if (e->row_rbmg >= 0)   // a_k at column k (ref rbmgaussian.cpp:19-22)
+        HIP_TRY(hipMemcpy(e->d_rbm_a, params + (size_t)e->row_rbmg * cols, nd * sizeof(double),
+                          hipMemcpyHostToDevice));
+    if (e->row_rbmp >= 0) {
+        // b_h at column h, W[k][h] at column H + k + nd*h (ref rbmproduct.cpp:18-27)
+        const double* row = params + (size_t)e->row_rbmp * cols;
+        std::vector<double> W((size_t)nd * H);
+        for (int k = 0; k < nd; k++)
+            for (int h = 0; h < H; h++) W[(size_t)k * H + h] = row[H + k + nd * h];
+        HIP_TRY(hipMemcpy(e->d_rbm_b, row, H * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->d_rbm_W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     e->params_set = true;
     return VMC_OK;
 }
@@ -327,6 +390,19 @@ static vmc::WalkParams base_params(const vmc_engine* e)
     P.q2_upper = (c.quirks & VMC_Q2_PADE_RATIO_UPPER) ? 1 : 0;
     P.q3_half = (c.quirks & VMC_Q3_PADE_GAMMA_HALF) ? 1 : 0;
     P.drift_a = c.drift_limit;
+    P.has_rbmg = e->row_rbmg >= 0;
+    P.has_rbmp = e->row_rbmp >= 0;
+    P.ND = e->nd;
+    P.H = P.has_rbmp ? c.num_hidden : 0;
+    P.sorting = c.sorting ? 1 : 0;
+    P.sig2_g = c.sigma * c.sigma;                              // ref rbmgaussian.cpp:16-17
+    P.isig2_p = 1.0 / ((10.0 * c.sigma) * (10.0 * c.sigma));  // ref rbmproduct.cpp:15-17
+    P.q13_psi_ratio = (c.quirks & VMC_Q13_RBMG_PSI_RATIO) ? 1 : 0;
+    P.q6_w_unscaled = (c.quirks & VMC_Q6_RBM_W_UNSCALED) ? 1 : 0;
+    P.rbm_a = e->d_rbm_a;
+    P.rbm_b = e->d_rbm_b;
+    P.rbm_W = e->d_rbm_W;
+    P.rbm_acc = e->d_rbm_acc;
     P.seed = e->seed;
     P.W = c.num_walkers;
     return P;
@@ -374,12 +450,19 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
         P.t_begin = t0;
         P.t_end = (t0 + seg < total) ? t0 + seg : total;
         P.acc_add = t0 > 0 ? 1 : 0;
-        HIP_TRY(vmc::launch_walk(e->cfg.precision, e->np, P, e->stream));
+        HIP_TRY(vmc::launch_walk(e->cfg.precision, e->shape, P, e->stream));
     }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     HIP_TRY(vmc::launch_reduce(e->d_acc, W, e->d_out, e->stream));
     double out[vmc::kNumWalkerAcc];
     HIP_TRY(hipMemcpyAsync(out, e->d_out, sizeof(out), hipMemcpyDeviceToHost, e->stream));
+    std::vector<double> rbm_out(e->rbm_width);
+    if (e->shape.rbm) {
+        HIP_TRY(vmc::launch_reduce_rows(e->d_rbm_acc, e->rbm_blocks, e->rbm_width, e->d_rbm_out,
+                                        e->stream));
+        HIP_TRY(hipMemcpyAsync(rbm_out.data(), e->d_rbm_out, sizeof(double) * e->rbm_width,
+                               hipMemcpyDeviceToHost, e->stream));
+    }
     if (rec_walkers > 0 && energies)
         HIP_TRY(hipMemcpyAsync(energies, e->d_rec, sizeof(float) * (size_t)rec_walkers * steps,
                                hipMemcpyDeviceToHost, e->stream));
@@ -417,6 +500,19 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
             if (grad) grad[e->row_pade * mp] = out[vmc::kAccGGamma];
             if (grad_e) grad_e[e->row_pade * mp] = out[vmc::kAccGGammaE];
         }
+        // RBM rows: the block-row layout is [a (nd) | b (H) | W (nd*H)] for the sums
+        // of dlnpsi/dtheta, then the same for E_L * dlnpsi/dtheta
+        const int nd = e->nd, H = e->shape.rbm ? base_params(e).H : 0;
+        const int half = nd + H + nd * H;
+        for (int hf = 0; hf < 2; hf++) {
+            double* g = hf ? grad_e : grad;
+            if (!g) continue;
+            if (e->row_rbmg >= 0)
+                for (int k = 0; k < nd; k++) g[e->row_rbmg * mp + k] = rbm_out[hf * half + k];
+            if (e->row_rbmp >= 0)
+                for (int c2 = 0; c2 < H + nd * H; c2++)
+                    g[e->row_rbmp * mp + c2] = rbm_out[hf * half + nd + c2];
+        }
     }
     return VMC_OK;
 }
@@ -436,8 +532,9 @@ int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration,
                     record_walkers, energies);
 }
 
-int vmc_evaluate(vmc_engine* e, const double* positions, int walkers, const int* moves,
-                 const double* new_values, double* out_before, double* out_after)
+static int evaluate_impl(vmc_engine* e, const double* positions, int walkers, const int* moves,
+                         const double* new_values, double* out_before, double* out_after,
+                         double* ratios, double* grads_before, double* grads_after)
 {
     if (!e || !positions || !moves || !new_values || !out_before || !out_after)
         return fail(VMC_E_INVALID, "null argument");
@@ -446,41 +543,88 @@ int vmc_evaluate(vmc_engine* e, const double* positions, int walkers, const int*
     for (int w = 0; w < walkers; w++)
         if (moves[w] < 0 || moves[w] >= e->nd) return fail(VMC_E_INVALID, "move index out of range");
     const int width = 9 + e->nd;
-    double *d_p = nullptr, *d_v = nullptr, *d_o0 = nullptr, *d_o1 = nullptr;
-    int* d_m = nullptr;
+    const int E = e->cfg.num_elements, gw = E * e->maxpar;
+    const bool want_g = grads_before && grads_after && gw > 0;
+    std::vector<double*> bufs;
     auto release = [&]() {
-        if (d_p) (void)hipFree(d_p);
-        if (d_v) (void)hipFree(d_v);
-        if (d_o0) (void)hipFree(d_o0);
-        if (d_o1) (void)hipFree(d_o1);
-        if (d_m) (void)hipFree(d_m);
+        for (double* b : bufs)
+            if (b) (void)hipFree(b);
     };
+    auto dalloc = [&](size_t n, double** out) {
+        *out = nullptr;
+        hipError_t err = hipMalloc(out, sizeof(double) * (n > 0 ? n : 1));
+        bufs.push_back(*out);
+        return err;
+    };
+    double *d_p, *d_v, *d_o0, *d_o1, *d_r = nullptr, *d_g0 = nullptr, *d_g1 = nullptr, *d_mi;
     hipError_t err;
     vmc::WalkParams P = base_params(e);
     P.W = walkers;
-    if ((err = hipMalloc(&d_p, sizeof(double) * walkers * e->nd)) != hipSuccess ||
-        (err = hipMalloc(&d_v, sizeof(double) * walkers)) != hipSuccess ||
-        (err = hipMalloc(&d_m, sizeof(int) * walkers)) != hipSuccess ||
-        (err = hipMalloc(&d_o0, sizeof(double) * walkers * width)) != hipSuccess ||
-        (err = hipMalloc(&d_o1, sizeof(double) * walkers * width)) != hipSuccess ||
-        (err = hipMemcpyAsync(d_p, positions, sizeof(double) * walkers * e->nd,
+    vmc::EvalOut O{};
+    O.num_elements = E;
+    O.maxpar = e->maxpar;
+    for (int i = 0; i < E; i++) O.rows[i] = e->cfg.kinds[i];
+    if ((err = dalloc((size_t)walkers * e->nd, &d_p)) != hipSuccess ||
+        (err = dalloc(walkers, &d_v)) != hipSuccess ||
+        (err = dalloc(walkers, &d_mi)) != hipSuccess ||
+        (err = dalloc((size_t)walkers * width, &d_o0)) != hipSuccess ||
+        (err = dalloc((size_t)walkers * width, &d_o1)) != hipSuccess ||
+        (ratios && (err = dalloc((size_t)walkers * E, &d_r)) != hipSuccess) ||
+        (want_g && (err = dalloc((size_t)walkers * gw, &d_g0)) != hipSuccess) ||
+        (want_g && (err = dalloc((size_t)walkers * gw, &d_g1)) != hipSuccess)) {
+        release();
+        return fail(VMC_E_DEVICE, std::string("evaluate: ") + hipGetErrorString(err));
+    }
+    int* d_m = reinterpret_cast<int*>(d_mi);
+    O.o0 = d_o0;
+    O.o1 = d_o1;
+    O.ratios = d_r;
+    O.g0 = d_g0;
+    O.g1 = d_g1;
+    if ((err = hipMemcpyAsync(d_p, positions, sizeof(double) * walkers * e->nd,
                               hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
         (err = hipMemcpyAsync(d_v, new_values, sizeof(double) * walkers, hipMemcpyHostToDevice,
                               e->stream)) != hipSuccess ||
         (err = hipMemcpyAsync(d_m, moves, sizeof(int) * walkers, hipMemcpyHostToDevice,
                               e->stream)) != hipSuccess ||
-        (err = vmc::launch_eval(e->cfg.precision, e->np, P, d_p, d_m, d_v, d_o0, d_o1,
-                                e->stream)) != hipSuccess ||
+        (want_g && (err = hipMemsetAsync(d_g0, 0, sizeof(double) * walkers * gw, e->stream)) !=
+                       hipSuccess) ||
+        (want_g && (err = hipMemsetAsync(d_g1, 0, sizeof(double) * walkers * gw, e->stream)) !=
+                       hipSuccess) ||
+        (err = vmc::launch_eval(e->cfg.precision, e->shape, P, d_p, d_m, d_v, O, e->stream)) !=
+            hipSuccess ||
         (err = hipMemcpyAsync(out_before, d_o0, sizeof(double) * walkers * width,
                               hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
         (err = hipMemcpyAsync(out_after, d_o1, sizeof(double) * walkers * width,
                               hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
+        (ratios && (err = hipMemcpyAsync(ratios, d_r, sizeof(double) * walkers * E,
+                                         hipMemcpyDeviceToHost, e->stream)) != hipSuccess) ||
+        (want_g && (err = hipMemcpyAsync(grads_before, d_g0, sizeof(double) * walkers * gw,
+                                         hipMemcpyDeviceToHost, e->stream)) != hipSuccess) ||
+        (want_g && (err = hipMemcpyAsync(grads_after, d_g1, sizeof(double) * walkers * gw,
+                                         hipMemcpyDeviceToHost, e->stream)) != hipSuccess) ||
         (err = hipStreamSynchronize(e->stream)) != hipSuccess) {
         release();
         return fail(VMC_E_DEVICE, std::string("evaluate: ") + hipGetErrorString(err));
     }
     release();
     return VMC_OK;
+}
+
+int vmc_evaluate(vmc_engine* e, const double* positions, int walkers, const int* moves,
+                 const double* new_values, double* out_before, double* out_after)
+{
+    return evaluate_impl(e, positions, walkers, moves, new_values, out_before, out_after, nullptr,
+                         nullptr, nullptr);
+}
+
+int vmc_evaluate_elements(vmc_engine* e, const double* positions, int walkers, const int* moves,
+                          const double* new_values, double* out_before, double* out_after,
+                          double* ratios, double* grads_before, double* grads_after)
+{
+    if (!ratios || !grads_before || !grads_after) return fail(VMC_E_INVALID, "null argument");
+    return evaluate_impl(e, positions, walkers, moves, new_values, out_before, out_after, ratios,
+                         grads_before, grads_after);
 }
 
 int vmc_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int n, int dof,

@@ -173,6 +173,12 @@ __device__ __forceinline__ double dpp(double v)
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+
 // sum over the K lanes of a group, every lane receives it.  Each step adds the
 // value of a partner lane, and partners pair up symmetrically, so all lanes
 // form the same floating-point sums.  Steps 1, 2, 4, 8 use DPP (after the
@@ -187,6 +193,19 @@ __device__ __forceinline__ T gsum(T v)
     if constexpr (K >= 16) v += dpp<kDppMirror>(v);
     if constexpr (K >= 32) v += __shfl_xor(v, 16, 64);
     if constexpr (K >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// product over the K lanes of a group (same pairing as gsum)
+template <int K, typename T>
+__device__ __forceinline__ T gprod(T v)
+{
+    if constexpr (K >= 2) v *= dpp<kDppXor1>(v);
+    if constexpr (K >= 4) v *= dpp<kDppXor2>(v);
+    if constexpr (K >= 8) v *= dpp<kDppHalfMirror>(v);
+    if constexpr (K >= 16) v *= dpp<kDppMirror>(v);
+    if constexpr (K >= 32) v *= __shfl_xor(v, 16, 64);
+    if constexpr (K >= 64) v *= __shfl_xor(v, 32, 64);
     return v;
 }
 
@@ -221,10 +240,11 @@ __device__ __forceinline__ void sfor(F&& f)
 // ------------------------------------------------------------------------
 // walker state
 // ------------------------------------------------------------------------
-template <typename R, int NP, int K>
+template <typename R, int NP, int K, bool RB>
 struct Walker {
     static constexpr int S = max_degree(NP);
     static constexpr int G = 64 / K;
+    static constexpr int HJ = 4;          // RBM hidden units per lane: h = l + K j, j < HJ
 
     // owned particles: slot 0 = particle l, slot 1 = particle NP + l
     R x[2][2];
@@ -243,6 +263,24 @@ struct Walker {
     bool valid[2];
     bool upspin[2];
     int l, gbase, g;
+
+    // RBM state (RB only).  Lane l owns hidden units h = l + K j; the sorted
+    // coordinate vector of the walker and its n = sigmoid(v) live in LDS.
+    R rv[HJ], rn[HJ], rp[HJ];   // pre-activation v, n, p = 1 - n of the current state
+    R pv[HJ], pn[HJ], pp[HJ];   // the same for the pending proposal
+    R rb[HJ], rc[HJ];           // b_h and sum_k W[k][h]^2
+    R ra[2][2];                 // RBM-Gaussian a of the owned coordinates
+    R gR[2][2];                 // RBM-product d ln psi / dx of the owned coordinates
+    int rk[2][2];               // rank of each owned coordinate in the sorted vector
+    double lr;                  // RBM-product Laplacian of ln psi (walker total)
+    int mk, r_old, r_new;       // the pending proposal: coordinate, its old and new rank
+    R* lW;                      // W[k][l' + K j] at lW[k * 4K + 4 l' + j]
+    R* xs_cur;                  // this walker's sorted coordinates
+    R* xs_prop;                 // the proposal's sorted coordinates (scratch)
+    R* ln;                      // this walker's n_h, h = 0..H-1
+
+    __device__ __forceinline__ bool hvalid(const WalkParams& P, int j) const { return l + K * j < P.H; }
+    __device__ __forceinline__ R isp2(const WalkParams& P) const { return R(P.isig2_p); }
 
     __device__ __forceinline__ R sw(const WalkParams& P) const { return R(::sqrt(P.omega)); }
 
@@ -426,6 +464,17 @@ struct Walker {
                 o0 -= oa * x[s][0];
                 o1 -= oa * x[s][1];
             }
+            if constexpr (RB) {
+                if (P.has_rbmp) {
+                    o0 += gR[s][0];
+                    o1 += gR[s][1];
+                }
+                if (P.has_rbmg) {
+                    const R is2 = R(1.0 / P.sig2_g);
+                    o0 -= (x[s][0] - ra[s][0]) * is2;
+                    o1 -= (x[s][1] - ra[s][1]) * is2;
+                }
+            }
             double v = double(o0) * double(o0) + double(o1) * double(o1);
             if (P.has_slater)
                 v += double(sS[s]) + 2.0 * (double(gS[s][0]) * double(o0) +
@@ -435,11 +484,167 @@ struct Walker {
         t = gsum<K>(t);
         if (P.has_pade) t += lj;
         if (P.has_gauss) t -= P.omega * P.alpha * 2.0 * P.N;
+        if constexpr (RB) {
+            if (P.has_rbmp) t += lr;
+            if (P.has_rbmg) t -= double(P.ND) / P.sig2_g;
+        }
         tk = -0.5 * t;
         vext = 0.5 * P.omega * P.omega * sumx2;
         el = tk + vext + vint;
         ga = P.has_gauss ? -0.5 * P.omega * sumx2 : 0.0;
         gg = P.has_pade ? (P.q3_half ? -0.5 : -1.0) * jg : 0.0;
+    }
+
+    // ---- RBM elements (ref src/WaveFunctions/rbmgaussian.cpp, rbmproduct.cpp) ----
+
+    // parameters of the lane's hidden units and owned coordinates; W into LDS
+    // (block-wide, before any other RBM call)
+    __device__ __forceinline__ void rbm_setup(const WalkParams& P)
+    {
+        const int ND = P.ND, H = P.H, K4 = 4 * K;
+        if (P.has_rbmp) {
+            for (int i = threadIdx.x; i < ND * K4; i += 64) {
+                const int k = i / K4, r = i - k * K4;
+                const int h = (r >> 2) + K * (r & 3);
+                lW[i] = h < H ? R(P.rbm_W[k * H + h]) : R(0);
+            }
+#pragma unroll
+            for (int j = 0; j < HJ; j++) {
+                const int h = l + K * j;
+                double c = 0;
+                if (h < H)
+                    for (int k = 0; k < ND; k++) c += P.rbm_W[k * H + h] * P.rbm_W[k * H + h];
+                rb[j] = h < H ? R(P.rbm_b[h]) : R(0);
+                rc[j] = R(c);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+                ra[s][d] = (P.has_rbmg && valid[s]) ? R(P.rbm_a[2 * (s * NP + l) + d]) : R(0);
+    }
+
+    // v, n, p of the lane's units from the current sorted vector
+    __device__ __forceinline__ void rbm_activate(const WalkParams& P)
+    {
+        const int K4 = 4 * K;
+        R acc[HJ];
+#pragma unroll
+        for (int j = 0; j < HJ; j++) acc[j] = R(0);
+        for (int i = 0; i < P.ND; i++) {
+            const R xi = xs_cur[i];
+#pragma unroll
+            for (int j = 0; j < HJ; j++) acc[j] += lW[i * K4 + 4 * l + j] * xi;
+        }
+#pragma unroll
+        for (int j = 0; j < HJ; j++) {
+            const bool ok = hvalid(P, j);
+            rv[j] = rb[j] + isp2(P) * acc[j];
+            const R e = M<R>::exp(rv[j]);
+            rp[j] = ok ? M<R>::rcp(R(1) + e) : R(0);
+            rn[j] = ok ? e * rp[j] : R(0);
+        }
+    }
+
+    // RBM-product gradient of the owned coordinates, d ln psi/dx_k =
+    // sum_h W[k][h] n_h / sigma_p^2 with the unsorted row k (ref rbmproduct.cpp:62-70),
+    // and the Laplacian sum_h (sum_k W[k][h]^2) n_h p_h / sigma_p^4 (ref :72-75).
+    // Reads the walker's n from LDS: call after a barrier that follows the staging.
+    __device__ __forceinline__ void rbm_refresh_grad(const WalkParams& P)
+    {
+        const int K4 = 4 * K;
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                R sum = R(0);
+                if (valid[s]) {
+                    const int k = 2 * (s * NP + l) + d;
+                    for (int h = 0; h < P.H; h++)
+                        sum += lW[k * K4 + 4 * (h & (K - 1)) + h / K] * ln[h];
+                }
+                gR[s][d] = isp2(P) * sum;
+            }
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < HJ; j++) t += double(rc[j]) * double(rn[j]) * double(rp[j]);
+        lr = P.isig2_p * P.isig2_p * gsum<K>(t);
+    }
+
+    __device__ __forceinline__ void rbm_stage_n(const WalkParams& P)
+    {
+#pragma unroll
+        for (int j = 0; j < HJ; j++)
+            if (hvalid(P, j)) ln[l + K * j] = rn[j];
+    }
+
+    // ranks of the owned coordinates, the sorted vector, v, n, p, gradients (block-wide)
+    __device__ __forceinline__ void rbm_init(const WalkParams& P)
+    {
+        if (!P.has_rbmp) return;
+        // stage the unsorted coordinates in the scratch row
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            if (valid[s]) {
+                xs_prop[2 * (s * NP + l)] = x[s][0];
+                xs_prop[2 * (s * NP + l) + 1] = x[s][1];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const int idx = 2 * (s * NP + l) + d;
+                int r = idx;
+                if (P.sorting && valid[s]) {
+                    // rank with ties broken by index (ref wavefunction.cpp:40-54 sorts values)
+                    r = 0;
+                    const R xv = x[s][d];
+                    for (int i = 0; i < P.ND; i++) {
+                        const R xi = xs_prop[i];
+                        r += (xi < xv || (xi == xv && i < idx)) ? 1 : 0;
+                    }
+                }
+                rk[s][d] = r;
+                if (valid[s]) xs_cur[r] = x[s][d];
+            }
+        __syncthreads();
+        rbm_activate(P);
+        rbm_stage_n(P);
+        __syncthreads();
+        rbm_refresh_grad(P);
+    }
+
+    // after a step (block-wide): commit an accepted proposal's RBM state,
+    // refresh the gradients and the local energy
+    __device__ __forceinline__ void rbm_after_step(const WalkParams& P, bool accepted)
+    {
+        if (P.has_rbmp) {
+            if (accepted) {
+                for (int i = l; i < P.ND; i += K) xs_cur[i] = xs_prop[i];
+#pragma unroll
+                for (int s = 0; s < 2; s++)
+#pragma unroll
+                    for (int d = 0; d < 2; d++) {
+                        const int idx = 2 * (s * NP + l) + d;
+                        int r = rk[s][d];
+                        r -= (r > r_old) ? 1 : 0;
+                        r += (r >= r_new) ? 1 : 0;
+                        rk[s][d] = (idx == mk) ? r_new : r;
+                    }
+#pragma unroll
+                for (int j = 0; j < HJ; j++) {
+                    rv[j] = pv[j];
+                    rn[j] = pn[j];
+                    rp[j] = pp[j];
+                }
+                rbm_stage_n(P);
+            }
+            __syncthreads();
+            if (accepted) rbm_refresh_grad(P);
+        }
+        if (accepted) local_energy(P);
     }
 };
 
@@ -450,13 +655,15 @@ struct Walker {
 template <typename R>
 struct StepInfo {
     R rs2, rp, rg;   // Slater |det'/det|^2, Pade and Gaussian ratios of the proposal
+    R rrg, rrp;      // RBM-Gaussian and RBM-product ratios
 };
 
-template <typename R, int NP, int K>
-__device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParams& P, int k, R z,
-                                            R um, R ua, bool forced, R xforced, StepInfo<R>& info)
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkParams& P, int k,
+                                            R z, R um, R ua, bool forced, R xforced,
+                                            StepInfo<R>& info)
 {
-    using WK = Walker<R, NP, K>;
+    using WK = Walker<R, NP, K, RB>;
     const R dt = R(P.dt), sqdt = R(::sqrt(P.dt));
     const R oa = R(P.omega * P.alpha);
     const R gam = R(P.gamma);
@@ -475,6 +682,14 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
     R mg = R(0);
     if (P.has_slater) mg += vsel(slot, vsel(d, w.gS[1][1], w.gS[1][0]), vsel(d, w.gS[0][1], w.gS[0][0]));
     if (P.has_pade) mg += vsel(slot, vsel(d, w.gJ[1][1], w.gJ[1][0]), vsel(d, w.gJ[0][1], w.gJ[0][0]));
+    if constexpr (RB) {
+        if (P.has_rbmp) mg += vsel(slot, vsel(d, w.gR[1][1], w.gR[1][0]), vsel(d, w.gR[0][1], w.gR[0][0]));
+        if (P.has_rbmg) {
+            const R xo = vsel(slot, vsel(d, w.x[1][1], w.x[1][0]), vsel(d, w.x[0][1], w.x[0][0]));
+            const R ao = vsel(slot, vsel(d, w.ra[1][1], w.ra[1][0]), vsel(d, w.ra[0][1], w.ra[0][0]));
+            mg -= (xo - ao) * R(1.0 / P.sig2_g);
+        }
+    }
     const R xpd = d ? xp1 : xp0;
     R Gd = __shfl(mg, osrc, 64);
     if (P.has_gauss) Gd -= oa * xpd;
@@ -576,11 +791,76 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
         if (exact) gsn = s * __shfl(d ? Ty : Tx, osrc, 64) * iR;
     }
 
+    // RBM ratios of the proposal (ref rbmgaussian.cpp:34-46, rbmproduct.cpp:43-52, 104-116)
+    R rrg = R(1), rrp = R(1), gRn = R(0), ak = R(0);
+    if constexpr (RB) {
+        if (P.has_rbmg) {
+            ak = __shfl(vsel(slot, vsel(d, w.ra[1][1], w.ra[1][0]), vsel(d, w.ra[0][1], w.ra[0][0])),
+                        osrc, 64);
+            const R xo = xpd - ak, xq = xnd - ak;
+            // the reference's ratio is psi'/psi, exp(d / (2 sigma^2)); |psi'/psi|^2 is exp(d / sigma^2)
+            const R den = R(P.q13_psi_ratio ? 0.5 / P.sig2_g : 1.0 / P.sig2_g);
+            rrg = M<R>::exp((xo * xo - xq * xq) * den);
+        }
+        if (P.has_rbmp) {
+            // the proposal's sorted vector: remove the moved value at its rank, insert
+            // the new one at r_new = #(other coordinates < x') (identity when unsorted)
+            int ro = k, rq = k;
+            if (P.sorting) {
+                ro = __shfl(vsel(slot, vsel(d, w.rk[1][1], w.rk[1][0]), vsel(d, w.rk[0][1], w.rk[0][0])),
+                            osrc, 64);
+                int cnt = 0;
+#pragma unroll
+                for (int s = 0; s < 2; s++)
+#pragma unroll
+                    for (int dd = 0; dd < 2; dd++) {
+                        const int idx = 2 * (s * NP + w.l) + dd;
+                        cnt += (w.valid[s] && idx != k && w.x[s][dd] < xnd) ? 1 : 0;
+                    }
+                rq = gsum<K>(cnt);
+            }
+            w.mk = k;
+            w.r_old = ro;
+            w.r_new = rq;
+            constexpr int K4 = 4 * K;
+            R acc[WK::HJ];
+#pragma unroll
+            for (int j = 0; j < WK::HJ; j++) acc[j] = R(0);
+            for (int i = 0; i < P.ND; i++) {
+                const int ii = i - (i > rq ? 1 : 0);
+                int src = ii + (ii >= ro ? 1 : 0);
+                src = src < P.ND ? src : P.ND - 1;
+                const R xs = w.xs_cur[src];
+                const R xi = (i == rq) ? xnd : xs;
+                if ((i & (K - 1)) == w.l) w.xs_prop[i] = xi;
+#pragma unroll
+                for (int j = 0; j < WK::HJ; j++) acc[j] += w.lW[i * K4 + 4 * w.l + j] * xi;
+            }
+            R q = R(1), gt = R(0);
+#pragma unroll
+            for (int j = 0; j < WK::HJ; j++) {
+                const bool ok = w.hvalid(P, j);
+                w.pv[j] = w.rb[j] + w.isp2(P) * acc[j];
+                const R e = M<R>::exp(w.pv[j]);
+                w.pp[j] = ok ? M<R>::rcp(R(1) + e) : R(0);
+                w.pn[j] = ok ? e * w.pp[j] : R(0);
+                // p_old / p_new = p_old (1 + e^{v'}) (ref rbmproduct.cpp:112-116)
+                q *= ok ? w.rp[j] * (R(1) + e) : R(1);
+                if (P.sampling == kImportanceExact) gt += w.lW[k * K4 + 4 * w.l + j] * w.pn[j];
+            }
+            q = gprod<K>(q);
+            rrp = q * q;
+            if (P.sampling == kImportanceExact) gRn = w.isp2(P) * gsum<K>(gt);
+        }
+    }
+
     // acceptance (ref importancesampling.cpp:43-47, 68-81; bruteforce.cpp:42-52)
     info.rs2 = Rs * Rs;
     info.rp = P.has_pade ? M<R>::exp(R(2) * jexp) : R(1);
     info.rg = P.has_gauss ? M<R>::exp(oa * (xpd * xpd - xnd * xnd)) : R(1);
-    const R ratio = info.rs2 * info.rp * info.rg;
+    info.rrg = rrg;
+    info.rrp = rrp;
+    const R ratio = info.rs2 * info.rp * info.rg * rrg * rrp;
     bool accept;
     if (forced) {
         accept = true;
@@ -594,6 +874,10 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
         if (P.has_slater) Gn += gsn;
         if (P.has_pade) Gn += d ? gp1 : gp0;
         if (P.has_gauss) Gn -= oa * xnd;
+        if constexpr (RB) {
+            if (P.has_rbmp) Gn += gRn;
+            if (P.has_rbmg) Gn -= (xnd - ak) * R(1.0 / P.sig2_g);
+        }
         const R Fr = R(2) * Gn;
         const R Fn =
             limit ? R(2) * Fr * M<R>::rcp(R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * Fr * Fr * dt))
@@ -644,12 +928,14 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K>& w, const WalkParam
             else { w.gS[0][0] = gx; w.gS[0][1] = gy; w.sS[0] = sec; }
         }
     }
-    w.local_energy(P);
+    // with RBM elements the state is completed (and E_L computed) block-wide
+    // in rbm_after_step
+    if constexpr (!RB) w.local_energy(P);
     return true;
 }
 
-template <typename R, int NP, int K>
-__device__ __forceinline__ void walker_load(Walker<R, NP, K>& w, const WalkParams& P,
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ void walker_load(Walker<R, NP, K, RB>& w, const WalkParams& P,
                                             const double* pos, const double* qf)
 {
 #pragma unroll
@@ -673,27 +959,72 @@ __device__ __forceinline__ void walker_load(Walker<R, NP, K>& w, const WalkParam
 }
 
 // full gradient d ln psi / dx of owned particle s, dimension dd
-template <typename R, int NP, int K>
-__device__ __forceinline__ R walker_gradient(const Walker<R, NP, K>& w, const WalkParams& P, int s,
-                                             int dd)
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ R walker_gradient(const Walker<R, NP, K, RB>& w, const WalkParams& P,
+                                             int s, int dd)
 {
     R g = R(0);
     if (P.has_slater) g += w.gS[s][dd];
     if (P.has_pade) g += w.gJ[s][dd];
     if (P.has_gauss) g -= R(P.omega * P.alpha) * w.x[s][dd];
+    if constexpr (RB) {
+        if (P.has_rbmp) g += w.gR[s][dd];
+        if (P.has_rbmg) g -= (w.x[s][dd] - w.ra[s][dd]) * R(1.0 / P.sig2_g);
+    }
     return g;
+}
+
+// ------------------------------------------------------------------------
+// RBM block-level LDS: gradient-window sums of the wave, then W and the
+// walkers' sorted coordinate vectors and n.  Sized at launch (dynamic LDS).
+// ------------------------------------------------------------------------
+template <typename R, int K>
+struct RbmLds {
+    double* accW;    // [2][ND*H] sum over the wave's walkers of xs_k n_h, and of E xs_k n_h
+    double* accAB;   // [2][ND + H] the a and b parts of the same sums
+    double* E;       // [G] staged local energies
+    R* W;            // [ND][4K]
+    R* xs;           // [G][2][ND] sorted coordinates: current, proposal
+    R* n;            // [G][H]
+    __device__ __forceinline__ void carve(double* base, int ND, int H)
+    {
+        constexpr int G = 64 / K;
+        accW = base;
+        accAB = accW + 2 * ND * H;
+        E = accAB + 2 * (ND + H);
+        W = reinterpret_cast<R*>(E + G);
+        xs = W + ND * 4 * K;
+        n = xs + G * 2 * ND;
+    }
+};
+
+inline size_t rbm_lds_bytes(int K, int ND, int H, size_t rsize)
+{
+    const int G = 64 / K;
+    return sizeof(double) * (2 * ND * H + 2 * (ND + H) + G) +
+           rsize * ((size_t)ND * 4 * K + (size_t)G * 2 * ND + (size_t)G * H);
+}
+
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& L, const WalkParams& P)
+{
+    w.lW = L.W;
+    w.xs_cur = L.xs + (size_t)w.g * 2 * P.ND;
+    w.xs_prop = w.xs_cur + P.ND;
+    w.ln = L.n + (size_t)w.g * P.H;
 }
 
 // ------------------------------------------------------------------------
 // the walk kernel: equil + steps Metropolis steps per walker
 // ------------------------------------------------------------------------
-template <typename R, int NP, int K>
+template <typename R, int NP, int K, bool RB>
 __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
 {
-    using WK = Walker<R, NP, K>;
+    using WK = Walker<R, NP, K, RB>;
     constexpr int G = WK::G;
     __shared__ double lds_T[G * NP * NP];
     __shared__ double lds_row[2 * G * NP];
+    extern __shared__ double rbm_dyn[];
 
     WK w;
     const int lane = threadIdx.x;
@@ -704,8 +1035,19 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
     const bool live = walker_raw < P.W;
     const int walker = live ? walker_raw : P.W - 1;
     const int nd = 2 * P.N;
+    const int glive = (P.W - (int)blockIdx.x * G) < G ? (P.W - (int)blockIdx.x * G) : G;
 
+    RbmLds<R, K> L;
     walker_load(w, P, P.pos + (size_t)walker * nd, P.qf ? P.qf + (size_t)walker * nd : nullptr);
+    if constexpr (RB) {
+        L.carve(rbm_dyn, P.ND, P.H);
+        rbm_bind(w, L, P);
+        for (int i = lane; i < 2 * P.ND * P.H; i += 64) L.accW[i] = 0.0;
+        for (int i = lane; i < 2 * (P.ND + P.H); i += 64) L.accAB[i] = 0.0;
+        w.rbm_setup(P);
+        __syncthreads();
+        w.rbm_init(P);
+    }
     if (P.has_slater) w.rebuild_slater(P, lds_T, lds_row);
     w.rebuild_pairs(P);
     w.local_energy(P);
@@ -719,6 +1061,10 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
 
     double aE = 0, aE2 = 0, aT = 0, aV = 0, aI = 0, aT2 = 0, aV2 = 0, aI2 = 0, aAcc = 0;
     double aGa = 0, aGg = 0, aGaE = 0, aGgE = 0;
+    double rGa[2][2] = {{0, 0}, {0, 0}}, rGaE[2][2] = {{0, 0}, {0, 0}};
+    double rGb[WK::HJ], rGbE[WK::HJ];
+#pragma unroll
+    for (int j = 0; j < WK::HJ; j++) rGb[j] = rGbE[j] = 0.0;
     int bad = 0;
 
     // random draws for K steps at a time, one step per lane of the group
@@ -738,6 +1084,7 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
 
         StepInfo<R> info;
         const bool accept = walker_step(w, P, k, z, um, ua, false, R(0), info);
+        if constexpr (RB) w.rbm_after_step(P, accept);
 
         if (t >= P.equil) {
             const double E = w.el;
@@ -756,9 +1103,109 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
                 aGg += w.gg;
                 aGaE += w.ga * E;
                 aGgE += w.gg * E;
+                if constexpr (RB) {
+                    // RBM-Gaussian: dlnpsi/da_k = (x_k - a_k)/sigma^2 (ref rbmgaussian.cpp:81-86)
+                    if (P.has_rbmg) {
+#pragma unroll
+                        for (int s = 0; s < 2; s++)
+#pragma unroll
+                            for (int d = 0; d < 2; d++) {
+                                const double xa =
+                                    w.valid[s] ? double(w.x[s][d] - w.ra[s][d]) / P.sig2_g : 0.0;
+                                rGa[s][d] += xa;
+                                rGaE[s][d] += E * xa;
+                            }
+                    }
+                    if (P.has_rbmp) {
+                        // dlnpsi/db_h = n_h (ref rbmproduct.cpp:77-83)
+#pragma unroll
+                        for (int j = 0; j < WK::HJ; j++) {
+                            rGb[j] += double(w.rn[j]);
+                            rGbE[j] += E * double(w.rn[j]);
+                        }
+                        // dlnpsi/dW[k][h] = xs_k n_h, summed over the wave's walkers: lane
+                        // (kb, hb) of an 8 x 8 grid owns a block of (k, h) pairs
+                        if (w.l == 0) L.E[w.g] = E;
+                        __syncthreads();
+                        const int kbs = (P.ND + 7) >> 3, hbs = (P.H + 7) >> 3;
+                        const int kb = lane >> 3, hb = lane & 7;
+                        for (int kk = 0; kk < kbs; kk++) {
+                            const int kc = kb * kbs + kk;
+                            if (kc >= P.ND) break;
+                            double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+                            for (int gg = 0; gg < glive; gg++) {
+                                const double xk = double(L.xs[(size_t)gg * 2 * P.ND + kc]);
+                                const double ex = L.E[gg] * xk;
+#pragma unroll
+                                for (int hh = 0; hh < 4; hh++) {
+                                    const int h = hb * hbs + hh;
+                                    if (hh < hbs && h < P.H) {
+                                        const double nv = double(L.n[(size_t)gg * P.H + h]);
+                                        s1[hh] += xk * nv;
+                                        s2[hh] += ex * nv;
+                                    }
+                                }
+                            }
+#pragma unroll
+                            for (int hh = 0; hh < 4; hh++) {
+                                const int h = hb * hbs + hh;
+                                if (hh < hbs && h < P.H) {
+                                    L.accW[kc * P.H + h] += s1[hh];
+                                    L.accW[P.ND * P.H + kc * P.H + h] += s2[hh];
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
             }
             if (P.rec && live && walker < P.rec_walkers && w.l == 0)
                 P.rec[(size_t)walker * P.steps + (t - P.equil)] = float(E);
+        }
+    }
+
+    if constexpr (RB) {
+        // the wave's a and b sums, walker by walker in a fixed order, then the
+        // block row of the gradient-window sums
+        const int nab = P.ND + P.H;
+        for (int gg = 0; gg < G; gg++) {
+            if (w.g == gg && live) {
+                if (P.has_rbmg) {
+#pragma unroll
+                    for (int s = 0; s < 2; s++)
+#pragma unroll
+                        for (int d = 0; d < 2; d++)
+                            if (w.valid[s]) {
+                                const int kc = 2 * (s * NP + w.l) + d;
+                                L.accAB[kc] += rGa[s][d];
+                                L.accAB[nab + kc] += rGaE[s][d];
+                            }
+                }
+                if (P.has_rbmp) {
+#pragma unroll
+                    for (int j = 0; j < WK::HJ; j++)
+                        if (w.hvalid(P, j)) {
+                            L.accAB[P.ND + w.l + K * j] += rGb[j];
+                            L.accAB[nab + P.ND + w.l + K * j] += rGbE[j];
+                        }
+                }
+            }
+            __syncthreads();
+        }
+        const int half = P.ND + P.H + P.ND * P.H;
+        const double wscale = P.q6_w_unscaled ? 1.0 : P.isig2_p;
+        double* row = P.rbm_acc + (size_t)blockIdx.x * rbm_row_width(P.ND, P.H);
+        for (int i = lane; i < 2 * half; i += 64) {
+            const int hf = i >= half ? 1 : 0;
+            const int r = i - hf * half;
+            double v;
+            if (r < nab) {
+                v = L.accAB[hf * nab + r];
+            } else {
+                const int m = r - nab, kc = m % P.ND, h = m / P.ND;
+                v = L.accW[hf * P.ND * P.H + kc * P.H + h] * wscale;
+            }
+            row[i] = P.acc_add ? row[i] + v : v;
         }
     }
 
@@ -787,17 +1234,53 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
 
 // ------------------------------------------------------------------------
 // parity kernel: evaluate, force one move through the incremental path,
-// evaluate again (see vmc_evaluate in include/vmc.h)
+// evaluate again (see vmc_evaluate and vmc_evaluate_elements in include/vmc.h)
 // ------------------------------------------------------------------------
-template <typename R, int NP, int K>
+
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ void eval_gradients(const Walker<R, NP, K, RB>& w, const WalkParams& P,
+                                               const EvalOut& O, double* g)
+{
+    const int mp = O.maxpar;
+    for (int e = 0; e < O.num_elements; e++) {
+        double* row = g + (size_t)e * mp;
+        const int kind = O.rows[e];
+        if (kind == 1 && w.l == 0 && mp > 0) row[0] = w.ga;          // Gaussian alpha
+        if (kind == 3 && w.l == 0 && mp > 0) row[0] = w.gg;          // Pade gamma
+        if constexpr (RB) {
+            if (kind == 4) {                                          // RBM-Gaussian a_k
+#pragma unroll
+                for (int s = 0; s < 2; s++)
+#pragma unroll
+                    for (int d = 0; d < 2; d++)
+                        if (w.valid[s])
+                            row[2 * (s * NP + w.l) + d] = double(w.x[s][d] - w.ra[s][d]) / P.sig2_g;
+            }
+            if (kind == 5) {                                          // RBM-product b_h, W[k][h]
+                const double ws = P.q6_w_unscaled ? 1.0 : P.isig2_p;
+#pragma unroll
+                for (int j = 0; j < Walker<R, NP, K, RB>::HJ; j++) {
+                    const int h = w.l + K * j;
+                    if (h >= P.H) continue;
+                    row[h] = double(w.rn[j]);
+                    for (int kc = 0; kc < P.ND; kc++)
+                        row[P.H + kc + P.ND * h] = double(w.xs_cur[kc]) * double(w.rn[j]) * ws;
+                }
+            }
+        }
+    }
+}
+
+template <typename R, int NP, int K, bool RB>
 __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const double* pos,
                                                   const int* moves, const double* newv,
-                                                  double* out0, double* out1)
+                                                  const EvalOut O)
 {
-    using WK = Walker<R, NP, K>;
+    using WK = Walker<R, NP, K, RB>;
     constexpr int G = WK::G;
     __shared__ double lds_T[G * NP * NP];
     __shared__ double lds_row[2 * G * NP];
+    extern __shared__ double rbm_dyn[];
     WK w;
     const int lane = threadIdx.x;
     w.l = lane & (K - 1);
@@ -808,16 +1291,25 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
     const int walker = live ? walker_raw : P.W - 1;
     const int nd = 2 * P.N;
     const int width = 9 + nd;
+    RbmLds<R, K> L;
     walker_load(w, P, pos + (size_t)walker * nd, nullptr);
+    if constexpr (RB) {
+        L.carve(rbm_dyn, P.ND, P.H);
+        rbm_bind(w, L, P);
+        w.rbm_setup(P);
+        __syncthreads();
+        w.rbm_init(P);
+    }
     if (P.has_slater) w.rebuild_slater(P, lds_T, lds_row);
     w.rebuild_pairs(P);
     w.local_energy(P);
     for (int pass = 0; pass < 2; pass++) {
-        StepInfo<R> info{R(0), R(0), R(0)};
+        StepInfo<R> info{R(0), R(0), R(0), R(0), R(0)};
         if (pass == 1) {
             walker_step(w, P, moves[walker], R(0), R(0), R(0), true, R(newv[walker]), info);
+            if constexpr (RB) w.rbm_after_step(P, true);
         }
-        double* o = (pass ? out1 : out0) + (size_t)walker * width;
+        double* o = (pass ? O.o1 : O.o0) + (size_t)walker * width;
         if (live && w.l == 0) {
             o[0] = w.el;
             o[1] = w.tk;
@@ -828,6 +1320,17 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
             o[6] = double(info.rs2);
             o[7] = double(info.rp);
             o[8] = double(info.rg);
+            if (pass == 1 && O.ratios) {
+                for (int e = 0; e < O.num_elements; e++) {
+                    const int kind = O.rows[e];
+                    const double r = kind == 1 ? double(info.rg)
+                                     : kind == 2 ? double(info.rs2)
+                                     : kind == 3 ? double(info.rp)
+                                     : kind == 4 ? double(info.rrg)
+                                                 : double(info.rrp);
+                    O.ratios[(size_t)walker * O.num_elements + e] = r;
+                }
+            }
         }
         if (live) {
 #pragma unroll
@@ -837,6 +1340,8 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
                 o[9 + 2 * q] = 2.0 * double(walker_gradient(w, P, s, 0));
                 o[9 + 2 * q + 1] = 2.0 * double(walker_gradient(w, P, s, 1));
             }
+            double* g = pass ? O.g1 : O.g0;
+            if (g) eval_gradients(w, P, O, g + (size_t)walker * O.num_elements * O.maxpar);
         }
     }
 }
@@ -852,23 +1357,25 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
 // ------------------------------------------------------------------------
 using RT = std::conditional_t<VMC_WALK_PRECISION == 32, float, double>;
 
-template <int NP, int K>
+template <int NP, int K, bool RB>
 static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
 {
     constexpr int G = 64 / K;
     const int blocks = (P.W + G - 1) / G;
-    hipLaunchKernelGGL((walk_kernel<RT, NP, K>), dim3(blocks), dim3(64), 0, stream, P);
+    const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT)) : 0;
+    hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB>), dim3(blocks), dim3(64), lds, stream, P);
     return hipGetLastError();
 }
 
-template <int NP, int K>
+template <int NP, int K, bool RB>
 static hipError_t launch_eval_t(const WalkParams& P, const double* pos, const int* moves,
-                                const double* newv, double* o0, double* o1, hipStream_t stream)
+                                const double* newv, const EvalOut& O, hipStream_t stream)
 {
     constexpr int G = 64 / K;
     const int blocks = (P.W + G - 1) / G;
-    hipLaunchKernelGGL((eval_kernel<RT, NP, K>), dim3(blocks), dim3(64), 0, stream, P, pos, moves,
-                       newv, o0, o1);
+    const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT)) : 0;
+    hipLaunchKernelGGL((eval_kernel<RT, NP, K, RB>), dim3(blocks), dim3(64), lds, stream, P, pos,
+                       moves, newv, O);
     return hipGetLastError();
 }
 
@@ -878,33 +1385,32 @@ static hipError_t launch_eval_t(const WalkParams& P, const double* pos, const in
 #define VMC_SUFFIX(name) name##_f64
 #endif
 
-hipError_t VMC_SUFFIX(launch_walk)(int np, const WalkParams& P, hipStream_t stream)
+// (np, K): Slater orbitals per spin (or particle slots per spin without a
+// determinant) and lanes per walker; rbm selects the RBM instantiations
+#define VMC_SHAPES(X)                                                        \
+    X(1, 1, false) X(3, 4, false) X(6, 8, false) X(10, 16, false)            \
+    X(15, 16, false) X(21, 32, false) X(28, 32, false)                       \
+    X(1, 1, true) X(1, 2, true) X(1, 4, true) X(1, 8, true) X(3, 4, true)    \
+    X(3, 8, true) X(6, 8, true) X(10, 16, true)
+
+hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, const WalkParams& P,
+                                   hipStream_t stream)
 {
-    switch (np) {
-    case 1: return launch_walk_t<1, 1>(P, stream);
-    case 3: return launch_walk_t<3, 4>(P, stream);
-    case 6: return launch_walk_t<6, 8>(P, stream);
-    case 10: return launch_walk_t<10, 16>(P, stream);
-    case 15: return launch_walk_t<15, 16>(P, stream);
-    case 21: return launch_walk_t<21, 32>(P, stream);
-    case 28: return launch_walk_t<28, 32>(P, stream);
-    }
+#define X(a, b, c) \
+    if (np == a && K == b && rbm == c) return launch_walk_t<a, b, c>(P, stream);
+    VMC_SHAPES(X)
+#undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t VMC_SUFFIX(launch_eval)(int np, const WalkParams& P, const double* pos,
-                                   const int* moves, const double* newv, double* o0, double* o1,
-                                   hipStream_t stream)
+hipError_t VMC_SUFFIX(launch_eval)(int np, int K, bool rbm, const WalkParams& P,
+                                   const double* pos, const int* moves, const double* newv,
+                                   const EvalOut& O, hipStream_t stream)
 {
-    switch (np) {
-    case 1: return launch_eval_t<1, 1>(P, pos, moves, newv, o0, o1, stream);
-    case 3: return launch_eval_t<3, 4>(P, pos, moves, newv, o0, o1, stream);
-    case 6: return launch_eval_t<6, 8>(P, pos, moves, newv, o0, o1, stream);
-    case 10: return launch_eval_t<10, 16>(P, pos, moves, newv, o0, o1, stream);
-    case 15: return launch_eval_t<15, 16>(P, pos, moves, newv, o0, o1, stream);
-    case 21: return launch_eval_t<21, 32>(P, pos, moves, newv, o0, o1, stream);
-    case 28: return launch_eval_t<28, 32>(P, pos, moves, newv, o0, o1, stream);
-    }
+#define X(a, b, c) \
+    if (np == a && K == b && rbm == c) return launch_eval_t<a, b, c>(P, pos, moves, newv, O, stream);
+    VMC_SHAPES(X)
+#undef X
     return hipErrorInvalidValue;
 }
 
@@ -970,21 +1476,25 @@ __global__ void init_positions_kernel(double* pos, int W, int nd, double omega, 
     if (2 * blk + 1 < nd) pos[(size_t)w * nd + 2 * blk + 1] = r * ::sin(th) / omega;
 }
 
-hipError_t launch_walk_f32(int np, const WalkParams& P, hipStream_t stream);
-hipError_t launch_walk_f64(int np, const WalkParams& P, hipStream_t stream);
-hipError_t launch_eval_f32(int np, const WalkParams& P, const double* pos, const int* moves,
-                           const double* newv, double* o0, double* o1, hipStream_t stream);
-hipError_t launch_eval_f64(int np, const WalkParams& P, const double* pos, const int* moves,
-                           const double* newv, double* o0, double* o1, hipStream_t stream);
+hipError_t launch_walk_f32(int np, int K, bool rbm, const WalkParams& P, hipStream_t stream);
+hipError_t launch_walk_f64(int np, int K, bool rbm, const WalkParams& P, hipStream_t stream);
+hipError_t launch_eval_f32(int np, int K, bool rbm, const WalkParams& P, const double* pos,
+                           const int* moves, const double* newv, const EvalOut& O,
+                           hipStream_t stream);
+hipError_t launch_eval_f64(int np, int K, bool rbm, const WalkParams& P, const double* pos,
+                           const int* moves, const double* newv, const EvalOut& O,
+                           hipStream_t stream);
 
 hipError_t launch_draws_f32(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
                             int* k, double* z, double* um, double* ua, hipStream_t stream);
 hipError_t launch_draws_f64(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
                             int* k, double* z, double* um, double* ua, hipStream_t stream);
 
-hipError_t launch_walk(int precision, int np, const WalkParams& P, hipStream_t stream)
+hipError_t launch_walk(int precision, const KernelShape& sh, const WalkParams& P,
+                       hipStream_t stream)
 {
-    return precision == 32 ? launch_walk_f32(np, P, stream) : launch_walk_f64(np, P, stream);
+    return precision == 32 ? launch_walk_f32(sh.np, sh.K, sh.rbm, P, stream)
+                           : launch_walk_f64(sh.np, sh.K, sh.rbm, P, stream);
 }
 
 hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int nd,
@@ -994,24 +1504,31 @@ hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t 
                            : launch_draws_f64(seed, walker, counter0, nd, n, k, z, um, ua, stream);
 }
 
-hipError_t launch_eval(int precision, int np, const WalkParams& P, const double* pos,
-                       const int* moves, const double* newv, double* o0, double* o1,
+hipError_t launch_eval(int precision, const KernelShape& sh, const WalkParams& P,
+                       const double* pos, const int* moves, const double* newv, const EvalOut& O,
                        hipStream_t stream)
 {
-    return precision == 32 ? launch_eval_f32(np, P, pos, moves, newv, o0, o1, stream)
-                           : launch_eval_f64(np, P, pos, moves, newv, o0, o1, stream);
+    return precision == 32 ? launch_eval_f32(sh.np, sh.K, sh.rbm, P, pos, moves, newv, O, stream)
+                           : launch_eval_f64(sh.np, sh.K, sh.rbm, P, pos, moves, newv, O, stream);
 }
 
-int walkers_per_wave(int np)
+// column sums of a row-major [rows][width] matrix, rows added in order
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const double* rows, int nrows, int width,
+                                                          double* out)
 {
-    switch (np) {
-    case 1: return 64;
-    case 3: return 16;
-    case 6: return 8;
-    case 10: case 15: return 4;
-    case 21: case 28: return 2;
-    }
-    return 0;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= width) return;
+    double s = 0;
+    for (int r = 0; r < nrows; r++) s += rows[(size_t)r * width + c];
+    out[c] = s;
+}
+
+hipError_t launch_reduce_rows(const double* rows, int nrows, int width, double* out,
+                              hipStream_t stream)
+{
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3((width + 255) / 256), dim3(256), 0, stream, rows,
+                       nrows, width, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream)

@@ -17,7 +17,8 @@ HEADER = os.path.join(ROOT, "include", "vmc.h")
 GAUSSIAN, SLATER, PADE, RBM_GAUSSIAN, RBM_PRODUCT = 1, 2, 3, 4, 5
 BRUTE_FORCE, IMPORTANCE, IMPORTANCE_EXACT = 0, 1, 2
 Q2_PADE_RATIO_UPPER, Q3_PADE_GAMMA_HALF = 1 << 2, 1 << 3
-QUIRKS_REFERENCE = Q2_PADE_RATIO_UPPER | Q3_PADE_GAMMA_HALF
+Q6_RBM_W_UNSCALED, Q13_RBMG_PSI_RATIO = 1 << 6, 1 << 13
+QUIRKS_REFERENCE = Q2_PADE_RATIO_UPPER | Q3_PADE_GAMMA_HALF | Q6_RBM_W_UNSCALED | Q13_RBMG_PSI_RATIO
 EVAL_E, EVAL_T, EVAL_VEXT, EVAL_VINT, EVAL_GALPHA, EVAL_GGAMMA = range(6)
 EVAL_RATIO_SLATER, EVAL_RATIO_PADE, EVAL_RATIO_GAUSS, EVAL_DRIFT = 6, 7, 8, 9
 
@@ -84,6 +85,8 @@ def lib():
         L.vmc_run_recorded.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, ctypes.POINTER(VmcSums),
                                        _DP, _DP, _I, _FP]
         L.vmc_evaluate.argtypes = [ctypes.c_void_p, _DP, _I, _IP, _DP, _DP, _DP]
+        L.vmc_evaluate_elements.argtypes = [ctypes.c_void_p, _DP, _I, _IP, _DP, _DP, _DP, _DP,
+                                            _DP, _DP]
         L.vmc_draws.argtypes = [_I, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _I, _I,
                                 _IP, _DP, _DP, _DP]
         _lib = L
@@ -233,3 +236,21 @@ class Engine:
         self._check(self.L.vmc_evaluate(self.h, _dp(pos), W, mv.ctypes.data_as(_IP), _dp(nv),
                                         _dp(o0), _dp(o1)))
         return o0, o1
+
+    def evaluate_elements(self, positions, moves, new_values):
+        """vmc_evaluate plus per-element ratios [W][E] of the forced move and the
+        parameter-gradient matrices [W][E][maxpar] before and after it."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        W = pos.shape[0]
+        mv = np.ascontiguousarray(moves, dtype=np.int32)
+        nv = np.ascontiguousarray(new_values, dtype=np.float64)
+        width = 9 + self.nd
+        o0, o1 = np.zeros((W, width)), np.zeros((W, width))
+        r = np.zeros((W, self.E))
+        g0 = np.zeros((W, self.E, max(self.maxpar, 1)))
+        g1 = np.zeros_like(g0)
+        self._check(self.L.vmc_evaluate_elements(self.h, _dp(pos), W, mv.ctypes.data_as(_IP),
+                                                 _dp(nv), _dp(o0), _dp(o1), _dp(r), _dp(g0),
+                                                 _dp(g1)))
+        mp = self.maxpar
+        return o0, o1, r, g0[:, :, :mp], g1[:, :, :mp]
