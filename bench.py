@@ -153,6 +153,48 @@ def cpu_baseline(args):
             "sample": f"oracle restatement (FP64), {W} walkers x {steps} steps on {cores} threads"}
 
 
+def rbmpj_workload(args, V):
+    """RBM x Slater x Pade-Jastrow, N=12, H=24, sigma=1, Xavier weights from the
+    seed (ref src/InitialWeights/xavier.cpp), reference importance sampling
+    dt=0.01, 65536 walkers: walker-steps/s of the FP64 and FP32 walks."""
+    N, H, nd = 12, 24, 24
+    rng = np.random.default_rng(args.seed)
+    a = 1.0 / np.sqrt(nd)
+    P = np.zeros((4, H + nd * H))
+    P[1, :nd] = rng.uniform(-a, a, nd)
+    P[2, :] = rng.uniform(-a, a, H + nd * H)
+    P[3, 0] = rng.uniform(0.0, 0.2)
+    res = {}
+    for prec in (32, 64):
+        cfg = V.make_config(N, omega=1.0, kinds=[V.SLATER, V.RBM_GAUSSIAN, V.RBM_PRODUCT, V.PADE],
+                            sampling=V.IMPORTANCE, step_length=0.01, precision=prec,
+                            refresh_interval=args.refresh if prec == 32 else 0,
+                            num_walkers=args.walkers, num_hidden=H, sigma=1.0, sorting=True)
+        eng = V.Engine(cfg)
+        eng.set_parameters(P)
+        eng.init_walkers(args.seed)
+        burn_in(eng, args.burnin, f"rbmpj fp{prec}")
+        iters = args.steps if prec == 32 else args.fp64_iters
+        wall, kern, E, acc = timed(eng, iters, args.warmup, args.steps_per_iter)
+        ws = args.walkers * args.steps_per_iter * iters
+        res[prec] = {"walker_steps_per_s": ws / wall, "kernel_walker_steps_per_s": ws / kern,
+                     "energy": E, "acceptance": acc, "ms_per_step": 1000.0 * wall / iters}
+        eng.close()
+    out = {"metric": "walker-steps/s, RBM x Slater x Pade-Jastrow N=12 H=24",
+           "value": res[32]["walker_steps_per_s"], "unit": "walker-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res[32]["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 walk, f64 inverse rebuild and accumulation",
+           "data": "synthetic (walkers drawn from N(0, 1/omega^2), burned in; Xavier weights)",
+           "config": {"workload": "2-D quantum dot N=12 omega=1, Coulomb, Slater x RBM-Gaussian x "
+                                  "RBM-product (H=24, sorted) x Pade-Jastrow, reference "
+                                  "importance sampling dt=0.01",
+                      "walkers": args.walkers, "steps_per_iter": args.steps_per_iter,
+                      "refresh_interval": args.refresh, "burnin_steps": args.burnin},
+           "fp32": res[32], "fp64": res[64]}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +218,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-iat", action="store_true")
+    ap.add_argument("--workload", choices=["dot56", "rbmpj"], default="dot56",
+                    help="dot56: the headline N=56 workload; rbmpj: RBM x Slater x Pade N=12 H=24")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -187,6 +231,9 @@ def main():
     if V.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "rbmpj":
+        rbmpj_workload(args, V)
+        return
     n = args.N // 2 if args.N > 2 else 1
     n = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28}[args.N]
 

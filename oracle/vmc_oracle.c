@@ -917,7 +917,7 @@ int vmco_step(vmco_walker* w, int k, double z, double u_move, double u_acc)
     double Fn = limited_drift(quantum_force(w, k), c->drift_limit, dt);
     double lg = -0.5 * dx * (F + Fn) + dt / 8.0 * (F * F - Fn * Fn);
     double a = p * exp(lg);
-    if (a < u_acc) { vmco_reject(w); return 0; }
+    if (!(a >= u_acc)) { vmco_reject(w); return 0; }   /* an undefined weight is rejected */
     return 1;
 }
 

@@ -867,8 +867,17 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
     } else if (P.sampling == kBruteForce) {
         accept = !(ratio < ua);
     } else if (P.sampling == kImportanceCompat) {
-        const R wgt = (M<R>::exp(R(0.5) * (stale - F) * dx) + R(1)) * ratio;
-        accept = !(wgt < ua);
+        const R xg = R(0.5) * (stale - F) * dx;
+        if constexpr (std::is_same<R, double>::value) {
+            // the reference's weight and test, `if (w < u) reject` (a NaN weight is accepted)
+            const R wgt = (M<R>::exp(xg) + R(1)) * ratio;
+            accept = !(wgt < ua);
+        } else {
+            // FP32: the same test in the log domain, log w = softplus(xg) + log ratio,
+            // which neither overflows nor forms inf x 0 next to a node
+            const R sp = (xg > R(0) ? xg : R(0)) + M<R>::log(R(1) + M<R>::exp(-(xg > R(0) ? xg : -xg)));
+            accept = sp + M<R>::log(ratio) >= M<R>::log(ua);
+        }
     } else {
         R Gn = R(0);
         if (P.has_slater) Gn += gsn;
@@ -883,9 +892,12 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
             limit ? R(2) * Fr * M<R>::rcp(R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * Fr * Fr * dt))
                   : Fr;
         const R lg = R(-0.5) * dx * (Fp + Fn) + dt * R(0.125) * (Fp * Fp - Fn * Fn);
-        accept = !(ratio * M<R>::exp(lg) < ua);
+        // an undefined weight (0 x inf next to a node) is rejected
+        if constexpr (std::is_same<R, double>::value)
+            accept = ratio * M<R>::exp(lg) >= ua;
+        else
+            accept = M<R>::log(ratio) + lg >= M<R>::log(ua);
     }
-    if (!forced && !(ratio == ratio)) accept = false;
     if (!accept) return false;
 
     const bool owner = (w.l == lp);
