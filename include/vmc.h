@@ -36,6 +36,7 @@ extern "C" {
 #define VMC_IMPORTANCE 1          /* sampling: importanceSampling, the reference's Green's-function ratio
                                      (ref Metropolis/importancesampling.cpp:43-47, 68-81) */
 #define VMC_IMPORTANCE_EXACT 2    /* Langevin proposal with the exact Metropolis-Hastings ratio */
+#define VMC_LEARNED 3             /* learned whole-electron proposal (vmc_lp_*), exact MH ratio */
 
 /* ---- reference behaviours that change results (set bit = reproduce) ---- */
 #define VMC_Q2_PADE_RATIO_UPPER (1u << 2)  /* ref padejastrow.cpp:214 */
@@ -173,6 +174,43 @@ int vmc_evaluate(vmc_engine* e, const double* positions, int walkers, const int*
 int vmc_evaluate_elements(vmc_engine* e, const double* positions, int walkers, const int* moves,
                           const double* new_values, double* out_before, double* out_after,
                           double* ratios, double* grads_before, double* grads_after);
+
+/* ---- Learned proposal (no counterpart in the reference) ----
+ * With sampling = VMC_LEARNED a step moves one whole electron i (drawn
+ * uniformly) by dx = mu + sigma * xi, xi ~ N(0, 1), where (mu, sigma) come from
+ * a per-electron MLP: 18 inputs (the electron's coordinates and the relative
+ * coordinates of its 8 nearest electrons, units of 1/sqrt(omega)), two tanh
+ * layers of 64, 4 outputs (mean, log-scale).  Acceptance is Metropolis-
+ * Hastings with the proposal ratio q(x|x')/q(x'|x) and the exact |psi'/psi|^2
+ * (the Pade ratio over all pairs whatever the quirk bits), so |psi|^2 stays
+ * the target.  The walk evaluates the network with bfloat16 weights and
+ * arithmetic in the walk precision.  Not available with RBM elements. */
+#define VMC_LP_NPARAM 5636   /* W1[64][18], b1[64], W2[64][64], b2[64], W3[4][64], b3[4] */
+typedef struct vmc_lp_hyper {
+    double sigma0;   /* move scale of the untrained (zero-head) network, units of 1/sqrt(omega) */
+    double mu_max;   /* |mean| < mu_max / sqrt(omega) */
+    double ls_max;   /* |log(scale / sigma0)| < ls_max */
+} vmc_lp_hyper;
+
+/* Network weights (FP32 master copy; the walk uses them rounded to bfloat16)
+ * and output bounds; resets the training state. */
+int vmc_lp_set(vmc_engine* e, const float* theta, int n, const vmc_lp_hyper* hp);
+int vmc_lp_get(vmc_engine* e, float* theta, int n);
+
+/* Parity entry point of the training kernels: for each configuration
+ * (positions [walkers][dof]), electron and normals [walkers][2], the proposal,
+ * J = A |dx|^2 omega with A = min(1, |psi'/psi|^2 q(x|x')/q(x'|x)), and dJ/dtheta.
+ * per_sample [walkers][5] = J, A, log|psi'/psi|^2, x', y'; grad [VMC_LP_NPARAM]
+ * = the sum over the configurations. */
+int vmc_lp_gradient(vmc_engine* e, const double* positions, int walkers, const int* electrons,
+                    const double* normals, double* per_sample, double* grad);
+
+/* Online training on the engine's own walkers: per iteration one sample per
+ * walker (electron and normals from Philox), gradient ascent on the mean J
+ * (ADAM, learning_rate), then walk_steps steps of the engine's sampling.
+ * history [iterations][3] (may be null) = mean J, mean A, mean |dx|^2 omega. */
+int vmc_lp_train(vmc_engine* e, int iterations, int walk_steps, double learning_rate,
+                 double* history);
 
 /* Parity entry point for the random-number protocol: the draws of steps
  * counter0 .. counter0+n-1 of one walker exactly as the walk computes them in

@@ -42,6 +42,60 @@ def build():
 
 _lib = None
 
+# learned proposal: network sizes and parameter offsets (vmc_oracle.h)
+LP_IN, LP_HID, LP_OUT = 18, 64, 4
+LP_W1 = 0
+LP_B1 = LP_W1 + LP_HID * LP_IN
+LP_W2 = LP_B1 + LP_HID
+LP_B2 = LP_W2 + LP_HID * LP_HID
+LP_W3 = LP_B2 + LP_HID
+LP_B3 = LP_W3 + LP_OUT * LP_HID
+LP_NPARAM = LP_B3 + LP_OUT
+
+
+class LpHyper(ctypes.Structure):
+    _fields_ = [("sigma0", ctypes.c_double), ("mu_max", ctypes.c_double),
+                ("ls_max", ctypes.c_double)]
+
+
+def lp_hyper(sigma0=0.2, mu_max=1.0, ls_max=2.0):
+    return LpHyper(sigma0, mu_max, ls_max)
+
+
+def lp_init(rng, zero_head=True):
+    """Glorot-uniform layers 1 and 2; the head zero (an untrained proposal is a
+    symmetric Gaussian random walk of width sigma0)."""
+    th = np.zeros(LP_NPARAM)
+    a1 = np.sqrt(6.0 / (LP_IN + LP_HID))
+    a2 = np.sqrt(6.0 / (2 * LP_HID))
+    th[LP_W1:LP_B1] = rng.uniform(-a1, a1, LP_HID * LP_IN)
+    th[LP_W2:LP_B2] = rng.uniform(-a2, a2, LP_HID * LP_HID)
+    if not zero_head:
+        a3 = np.sqrt(6.0 / (LP_HID + LP_OUT))
+        th[LP_W3:LP_B3] = rng.uniform(-a3, a3, LP_OUT * LP_HID)
+    return th
+
+
+def lp_features(cfg, x, i, xi, yi):
+    f = np.zeros(LP_IN)
+    nn = lib().vmco_lp_features(ctypes.byref(cfg), _dp(np.ascontiguousarray(x, dtype=np.float64)),
+                                int(i), float(xi), float(yi), _dp(f))
+    return f, nn
+
+
+def lp_forward(theta, f):
+    h1, h2, o = np.zeros(LP_HID), np.zeros(LP_HID), np.zeros(LP_OUT)
+    lib().vmco_lp_forward(_dp(np.ascontiguousarray(theta, dtype=np.float64)),
+                          _dp(np.ascontiguousarray(f, dtype=np.float64)), _dp(h1), _dp(h2), _dp(o))
+    return h1, h2, o
+
+
+def lp_head(hp, omega, o):
+    mu, sig = np.zeros(2), np.zeros(2)
+    lib().vmco_lp_head(ctypes.byref(hp), omega, _dp(np.ascontiguousarray(o, dtype=np.float64)),
+                       _dp(mu), _dp(sig))
+    return mu, sig
+
 
 def lib():
     global _lib
@@ -97,6 +151,14 @@ def lib():
         for name in ["vmco_basis_element_der", "vmco_basis_element_secder"]:
             getattr(L, name).argtypes = [_P, _I, _I, _DP]
             getattr(L, name).restype = _D
+        L.vmco_lp_features.argtypes = [ctypes.POINTER(Config), _DP, _I, _D, _D, _DP]
+        L.vmco_lp_features.restype = _I
+        L.vmco_lp_forward.argtypes = [_DP, _DP, _DP, _DP, _DP]
+        L.vmco_lp_head.argtypes = [ctypes.POINTER(LpHyper), _D, _DP, _DP, _DP]
+        L.vmco_lp_step.argtypes = [_P, _DP, ctypes.POINTER(LpHyper), _I, _D, _D, _D]
+        L.vmco_lp_step.restype = _I
+        L.vmco_lp_sample_grad.argtypes = [_P, _DP, ctypes.POINTER(LpHyper), _I, _D, _D, _DP, _DP]
+        L.vmco_lp_sample_grad.restype = _D
         _lib = L
     return _lib
 
@@ -215,6 +277,22 @@ class Walker:
 
     def step(self, k, z, u_move, u_acc):
         return bool(self.L.vmco_step(self.h, int(k), float(z), float(u_move), float(u_acc)))
+
+    def lp_step(self, theta, hp, i, xi0, xi1, ua):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        return bool(self.L.vmco_lp_step(self.h, _dp(th), ctypes.byref(hp), int(i), float(xi0),
+                                        float(xi1), float(ua)))
+
+    def lp_sample_grad(self, theta, hp, i, xi0, xi1, grad=None):
+        """J = A |dx|^2 omega of one proposal; adds dJ/dtheta into grad.
+        Returns (J, A, log|psi'/psi|^2, x', y', grad)."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        g = np.zeros(LP_NPARAM) if grad is None else grad
+        out = np.zeros(6)
+        self.L.vmco_lp_sample_grad(self.h, _dp(th), ctypes.byref(hp), int(i), float(xi0),
+                                   float(xi1), _dp(g), _dp(out))
+        self.last_L = out[5]
+        return out[0], out[1], out[2], out[3], out[4], g
 
     def run(self, seed, walker, counter, steps, equil, it=0, batches=1, record=False):
         E = self.cfg.num_elements

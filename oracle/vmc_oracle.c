@@ -1157,3 +1157,222 @@ int vmco_blocking(double* x, uint64_t n, double* mean, double* stderr_, uint64_t
     free(gam); free(s); free(X); free(M); free(cs);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- *
+ * Learned Metropolis proposal (no reference counterpart; see vmc_oracle.h)
+ * ------------------------------------------------------------------------- */
+int vmco_lp_features(const vmco_config* c, const double* x, int i, double xi, double yi, double* f)
+{
+    const double s = sqrt(c->omega);
+    int N = c->N;
+    for (int m = 0; m < VMCO_LP_IN; m++) f[m] = 0.0;
+    f[0] = s * xi;
+    f[1] = s * yi;
+    /* the 8 nearest other electrons by squared distance, ties to the lower index */
+    int taken[64] = {0};
+    int nn = N - 1 < 8 ? N - 1 : 8;
+    for (int m = 0; m < nn; m++) {
+        int best = -1;
+        double bd = 0;
+        for (int j = 0; j < N; j++) {
+            if (j == i || taken[j]) continue;
+            double dx = x[2 * j] - xi, dy = x[2 * j + 1] - yi;
+            double d2 = dx * dx + dy * dy;
+            if (best < 0 || d2 < bd) { best = j; bd = d2; }
+        }
+        taken[best] = 1;
+        f[2 + 2 * m] = s * (x[2 * best] - xi);
+        f[3 + 2 * m] = s * (x[2 * best + 1] - yi);
+    }
+    return nn;
+}
+
+void vmco_lp_forward(const double* th, const double* f, double* h1, double* h2, double* o)
+{
+    double a1[VMCO_LP_HID], a2[VMCO_LP_HID];
+    for (int u = 0; u < VMCO_LP_HID; u++) {
+        double a = th[VMCO_LP_B1 + u];
+        for (int m = 0; m < VMCO_LP_IN; m++) a += th[VMCO_LP_W1 + u * VMCO_LP_IN + m] * f[m];
+        a1[u] = tanh(a);
+    }
+    for (int u = 0; u < VMCO_LP_HID; u++) {
+        double a = th[VMCO_LP_B2 + u];
+        for (int c = 0; c < VMCO_LP_HID; c++) a += th[VMCO_LP_W2 + u * VMCO_LP_HID + c] * a1[c];
+        a2[u] = tanh(a);
+    }
+    for (int r = 0; r < VMCO_LP_OUT; r++) {
+        double a = th[VMCO_LP_B3 + r];
+        for (int u = 0; u < VMCO_LP_HID; u++) a += th[VMCO_LP_W3 + r * VMCO_LP_HID + u] * a2[u];
+        o[r] = a;
+    }
+    if (h1) memcpy(h1, a1, sizeof(a1));
+    if (h2) memcpy(h2, a2, sizeof(a2));
+}
+
+/* backward of the network: g_o = dX/do; adds scale * dX/dtheta into grad and
+ * returns dX/df (unscaled) in g_f if non-NULL */
+static void lp_backward(const double* th, const double* f, const double* h1, const double* h2,
+                        const double* g_o, double scale, double* grad, double* g_f)
+{
+    double ga2[VMCO_LP_HID], ga1[VMCO_LP_HID];
+    for (int u = 0; u < VMCO_LP_HID; u++) {
+        double g = 0;
+        for (int r = 0; r < VMCO_LP_OUT; r++) g += th[VMCO_LP_W3 + r * VMCO_LP_HID + u] * g_o[r];
+        ga2[u] = g * (1.0 - h2[u] * h2[u]);
+    }
+    for (int r = 0; r < VMCO_LP_OUT; r++) {
+        for (int u = 0; u < VMCO_LP_HID; u++) grad[VMCO_LP_W3 + r * VMCO_LP_HID + u] += scale * g_o[r] * h2[u];
+        grad[VMCO_LP_B3 + r] += scale * g_o[r];
+    }
+    for (int c = 0; c < VMCO_LP_HID; c++) {
+        double g = 0;
+        for (int u = 0; u < VMCO_LP_HID; u++) g += th[VMCO_LP_W2 + u * VMCO_LP_HID + c] * ga2[u];
+        ga1[c] = g * (1.0 - h1[c] * h1[c]);
+    }
+    for (int u = 0; u < VMCO_LP_HID; u++) {
+        for (int c = 0; c < VMCO_LP_HID; c++) grad[VMCO_LP_W2 + u * VMCO_LP_HID + c] += scale * ga2[u] * h1[c];
+        grad[VMCO_LP_B2 + u] += scale * ga2[u];
+    }
+    for (int c = 0; c < VMCO_LP_HID; c++) {
+        for (int m = 0; m < VMCO_LP_IN; m++) grad[VMCO_LP_W1 + c * VMCO_LP_IN + m] += scale * ga1[c] * f[m];
+        grad[VMCO_LP_B1 + c] += scale * ga1[c];
+    }
+    if (g_f) {
+        for (int m = 0; m < VMCO_LP_IN; m++) {
+            double g = 0;
+            for (int c = 0; c < VMCO_LP_HID; c++) g += th[VMCO_LP_W1 + c * VMCO_LP_IN + m] * ga1[c];
+            g_f[m] = g;
+        }
+    }
+}
+
+void vmco_lp_head(const vmco_lp_hyper* hp, double omega, const double* o, double* mu, double* sig)
+{
+    const double ell = 1.0 / sqrt(omega);
+    for (int d = 0; d < 2; d++) {
+        mu[d] = ell * hp->mu_max * tanh(o[d] / hp->mu_max);
+        sig[d] = hp->sigma0 * ell * exp(hp->ls_max * tanh(o[2 + d] / hp->ls_max));
+    }
+}
+
+/* log|psi(x')/psi(x)|^2 of moving electron i to (xn, yn) as two coordinate
+ * moves, and the drift 2 grad ln|psi| of electron i at x'; the walker is left
+ * at x' */
+static double lp_move(vmco_walker* w, int i, double xn, double yn, double* F)
+{
+    vmco_propose(w, 2 * i, xn);
+    double r1 = vmco_probability_ratio(w);
+    vmco_accept(w);
+    vmco_propose(w, 2 * i + 1, yn);
+    double r2 = vmco_probability_ratio(w);
+    vmco_accept(w);
+    if (F) {
+        F[0] = quantum_force(w, 2 * i);
+        F[1] = quantum_force(w, 2 * i + 1);
+    }
+    return log(r1) + log(r2);
+}
+
+typedef struct {
+    double fa[VMCO_LP_IN], h1a[VMCO_LP_HID], h2a[VMCO_LP_HID], oa[VMCO_LP_OUT];
+    double fb[VMCO_LP_IN], h1b[VMCO_LP_HID], h2b[VMCO_LP_HID], ob[VMCO_LP_OUT];
+    double mu[2], sig[2], mub[2], sigb[2], dx[2], r[2];
+    double xn, yn, lqf, lqr;
+    int nnb;
+} lp_pass;
+
+/* both network passes of a proposal of electron i with normals xi */
+static void lp_propose(const vmco_walker* w, const double* th, const vmco_lp_hyper* hp, int i,
+                       const double* xi, lp_pass* q)
+{
+    const double* x = w->x;
+    vmco_lp_features(&w->c, x, i, x[2 * i], x[2 * i + 1], q->fa);
+    vmco_lp_forward(th, q->fa, q->h1a, q->h2a, q->oa);
+    vmco_lp_head(hp, w->c.omega, q->oa, q->mu, q->sig);
+    q->lqf = 0;
+    for (int d = 0; d < 2; d++) {
+        q->dx[d] = q->mu[d] + q->sig[d] * xi[d];
+        q->lqf -= 0.5 * xi[d] * xi[d] + log(q->sig[d]);
+    }
+    q->xn = x[2 * i] + q->dx[0];
+    q->yn = x[2 * i + 1] + q->dx[1];
+    q->nnb = vmco_lp_features(&w->c, x, i, q->xn, q->yn, q->fb);
+    vmco_lp_forward(th, q->fb, q->h1b, q->h2b, q->ob);
+    vmco_lp_head(hp, w->c.omega, q->ob, q->mub, q->sigb);
+    q->lqr = 0;
+    for (int d = 0; d < 2; d++) {
+        q->r[d] = -q->dx[d] - q->mub[d];
+        q->lqr -= q->r[d] * q->r[d] / (2.0 * q->sigb[d] * q->sigb[d]) + log(q->sigb[d]);
+    }
+}
+
+int vmco_lp_step(vmco_walker* w, const double* th, const vmco_lp_hyper* hp, int i, double xi0,
+                 double xi1, double ua)
+{
+    const double xi[2] = {xi0, xi1};
+    lp_pass q;
+    lp_propose(w, th, hp, i, xi, &q);
+    double* old = dalloc(w->ND);
+    memcpy(old, w->x, sizeof(double) * w->ND);
+    double logR = lp_move(w, i, q.xn, q.yn, NULL);
+    int acc = (logR + q.lqr - q.lqf >= log(ua)) ? 1 : 0;
+    if (!acc) vmco_init_positions(w, old);
+    free(old);
+    return acc;
+}
+
+double vmco_lp_sample_grad(vmco_walker* w, const double* th, const vmco_lp_hyper* hp, int i,
+                           double xi0, double xi1, double* grad, double* out)
+{
+    const double xi[2] = {xi0, xi1};
+    const double omega = w->c.omega, s = sqrt(omega), ell = 1.0 / s;
+    lp_pass q;
+    lp_propose(w, th, hp, i, xi, &q);
+    double* old = dalloc(w->ND);
+    memcpy(old, w->x, sizeof(double) * w->ND);
+    double F[2];
+    double logR = lp_move(w, i, q.xn, q.yn, F);
+    vmco_init_positions(w, old);
+    free(old);
+
+    double L = logR + q.lqr - q.lqf;
+    double A = L < 0 ? exp(L) : 1.0;
+    double d2 = (q.dx[0] * q.dx[0] + q.dx[1] * q.dx[1]) * omega;
+    double J = A * d2;
+    double gL = (L < 0 ? A : 0.0) * d2;                 /* dJ/dL */
+    /* d log q_r / d o_b */
+    double gob[VMCO_LP_OUT], gfb[VMCO_LP_IN];
+    for (int d = 0; d < 2; d++) {
+        double t = tanh(q.ob[d] / hp->mu_max), ts = tanh(q.ob[2 + d] / hp->ls_max);
+        double is2 = 1.0 / (q.sigb[d] * q.sigb[d]);
+        gob[d] = q.r[d] * is2 * ell * (1.0 - t * t);
+        gob[2 + d] = (q.r[d] * q.r[d] * is2 - 1.0) * (1.0 - ts * ts);
+    }
+    lp_backward(th, q.fb, q.h1b, q.h2b, gob, gL, grad, gfb);
+    /* dJ/d dx: explicit |dx|^2, and through L (psi at x', q_r explicit and via f_b) */
+    double gdx[2];
+    for (int d = 0; d < 2; d++) {
+        double viaf = gfb[d];
+        for (int m = 0; m < q.nnb; m++) viaf -= gfb[2 + 2 * m + d];
+        double dLdx = F[d] + q.r[d] / (q.sigb[d] * q.sigb[d]) + s * viaf;
+        gdx[d] = A * 2.0 * q.dx[d] * omega + gL * dLdx;
+    }
+    double goa[VMCO_LP_OUT];
+    for (int d = 0; d < 2; d++) {
+        double t = tanh(q.oa[d] / hp->mu_max), ts = tanh(q.oa[2 + d] / hp->ls_max);
+        double dJdmu = gdx[d];
+        double dJdsig = gdx[d] * xi[d] + gL / q.sig[d];
+        goa[d] = dJdmu * ell * (1.0 - t * t);
+        goa[2 + d] = dJdsig * q.sig[d] * (1.0 - ts * ts);
+    }
+    lp_backward(th, q.fa, q.h1a, q.h2a, goa, 1.0, grad, NULL);
+    if (out) {
+        out[0] = J;
+        out[1] = A;
+        out[2] = logR;
+        out[3] = q.xn;
+        out[4] = q.yn;
+        out[5] = L;
+    }
+    return J;
+}

@@ -40,7 +40,7 @@ enum {
 enum {
     VMCO_Q2_PADE_RATIO_UPPER = 1u << 2,  /* padejastrow.cpp:214: only pairs j > i_p in the ratio */
     VMCO_Q3_PADE_GAMMA_HALF = 1u << 3,   /* padejastrow.cpp:114: extra 1/2 in dlnpsi/dgamma     */
-    VMCO_Q6_RBM_W_NO_SIGMA = 1u << 6,    /* rbmproduct.cpp:98: dlnpsi/dW lacks 1/sigma_p^2       */
+    VMCO_Q6_RBM_W_NO_SIGMA = 1u << 6,    /* rbmproduct.cpp:77-83: dlnpsi/dW lacks 1/sigma_p^2    */
     VMCO_Q13_RBMG_SQRT_RATIO = 1u << 13, /* rbmgaussian.cpp:44-45: ratio uses 2 sigma^2            */
     VMCO_QUIRKS_ALL = (1u << 2) | (1u << 3) | (1u << 6) | (1u << 13)
 };
@@ -157,6 +157,55 @@ int vmco_blocking(double* x, uint64_t n, double* mean, double* stderr_, uint64_t
 double vmco_chi2_quantile95(int dof);
 
 /* ---- Hermite basis (Basis/hermite.cpp, Basis/basis.cpp) ---- */
+/* ---- learned Metropolis proposal (no reference counterpart) ----
+ * A per-electron MLP maps features of the moved electron (its own coordinates
+ * and the relative coordinates of its 8 nearest electrons, in units of
+ * 1/sqrt(omega)) through two tanh layers of width 64 to a Gaussian move mean
+ * and a diagonal scale.  The proposal moves one whole electron; acceptance is
+ * Metropolis-Hastings with the proposal ratio.  The network is trained to
+ * maximise the expected squared jump distance E[A |dx|^2].  Pins: finite
+ * differences of the gradient, the zero-network closed form (a symmetric
+ * Gaussian random walk, q ratio 1). */
+#define VMCO_LP_IN 18
+#define VMCO_LP_HID 64
+#define VMCO_LP_OUT 4
+#define VMCO_LP_NPARAM (VMCO_LP_HID * VMCO_LP_IN + VMCO_LP_HID + VMCO_LP_HID * VMCO_LP_HID + \
+                        VMCO_LP_HID + VMCO_LP_OUT * VMCO_LP_HID + VMCO_LP_OUT)
+/* parameter offsets: W1[u][m], b1[u], W2[u][c], b2[u], W3[r][u], b3[r] */
+#define VMCO_LP_W1 0
+#define VMCO_LP_B1 (VMCO_LP_W1 + VMCO_LP_HID * VMCO_LP_IN)
+#define VMCO_LP_W2 (VMCO_LP_B1 + VMCO_LP_HID)
+#define VMCO_LP_B2 (VMCO_LP_W2 + VMCO_LP_HID * VMCO_LP_HID)
+#define VMCO_LP_W3 (VMCO_LP_B2 + VMCO_LP_HID)
+#define VMCO_LP_B3 (VMCO_LP_W3 + VMCO_LP_OUT * VMCO_LP_HID)
+
+typedef struct vmco_lp_hyper {
+    double sigma0;   /* scale of the untrained proposal, in units of 1/sqrt(omega) */
+    double mu_max;   /* |mean| < mu_max / sqrt(omega) */
+    double ls_max;   /* |log(scale / sigma0)| < ls_max */
+} vmco_lp_hyper;
+
+/* features of electron i placed at (xi, yi), all other electrons at x; returns
+ * the number of neighbours used (min(8, N-1)) */
+int vmco_lp_features(const vmco_config* c, const double* x, int i, double xi, double yi,
+                     double* f /* 18 */);
+/* network forward: h1, h2 (64 each, may be NULL), o (4) */
+void vmco_lp_forward(const double* theta, const double* f, double* h1, double* h2, double* o);
+/* move mean and scale of the head outputs o */
+void vmco_lp_head(const vmco_lp_hyper* hp, double omega, const double* o, double* mu, double* sig);
+/* one learned-proposal step of the walker: electron i, normals xi0, xi1,
+ * acceptance uniform ua.  The walker's configuration must use exact ratios
+ * (quirks 0).  Returns 1 if accepted. */
+int vmco_lp_step(vmco_walker* w, const double* theta, const vmco_lp_hyper* hp, int i, double xi0,
+                 double xi1, double ua);
+/* one training sample at the walker's current configuration: J = A |dx|^2 omega
+ * and, added into grad (VMCO_LP_NPARAM), dJ/dtheta.  Also returns log|psi'/psi|^2,
+ * the acceptance A, the proposed position and the log weight L (A = min(1, e^L)).
+ * The walker is left unchanged. */
+double vmco_lp_sample_grad(vmco_walker* w, const double* theta, const vmco_lp_hyper* hp, int i,
+                           double xi0, double xi1, double* grad,
+                           double* out /* 6: J, A, logR, x', y', L */);
+
 int vmco_num_orbitals(int N, int D);  /* n = N/2 for closed shells, -1 if open */
 void vmco_orbital_list(int n, int D, int* states /* n*D */);
 /* value / derivative / second derivative of orbital j along dimension d at r */

@@ -5,7 +5,20 @@
 namespace vmc {
 
 // sampling methods (same values as include/vmc.h)
-enum : int { kBruteForce = 0, kImportanceCompat = 1, kImportanceExact = 2 };
+enum : int { kBruteForce = 0, kImportanceCompat = 1, kImportanceExact = 2, kLearned = 3 };
+
+// learned proposal network: 18 inputs, two tanh layers of 64, 4 outputs;
+// parameters W1[u][m], b1[u], W2[u][c], b2[u], W3[r][u], b3[r]
+enum : int {
+    kLpIn = 18, kLpHid = 64, kLpOut = 4,
+    kLpW1 = 0,
+    kLpB1 = kLpW1 + kLpHid * kLpIn,
+    kLpW2 = kLpB1 + kLpHid,
+    kLpB2 = kLpW2 + kLpHid * kLpHid,
+    kLpW3 = kLpB2 + kLpHid,
+    kLpB3 = kLpW3 + kLpOut * kLpHid,
+    kLpParams = kLpB3 + kLpOut
+};
 
 // per-walker accumulator slots written by the walk kernel
 enum : int {
@@ -43,6 +56,11 @@ struct WalkParams {
     const double* rbm_b;    // [H]
     const double* rbm_W;    // [ND][H], W[k][h] at k*H + h
     double* rbm_acc;        // [blocks][rbm_row_width(ND, H)] gradient-window sums per block
+    // learned proposal (sampling == kLearned)
+    const uint16_t* lp_w;   // [kLpParams] network weights, bfloat16
+    double lp_sigma0;       // untrained move scale, units of 1/sqrt(omega)
+    double lp_mu_max;       // |move mean| < lp_mu_max / sqrt(omega)
+    double lp_ls_max;       // |log(scale / sigma0)| < lp_ls_max
     // Metropolis
     int sampling;
     double dt;          // importance dt or brute-force step length
@@ -71,8 +89,27 @@ struct WalkParams {
     int init_qf;        // 1: set the stored drift to the drift at the loaded positions
 };
 
+// training of the learned proposal: one sample per walker (vmc_learn.hip)
+struct LpTrainParams {
+    int B, N;               // samples, electrons
+    double omega;
+    uint64_t seed;
+    double sigma0, mu_max, ls_max;
+    const float* theta;     // [kLpParams] FP32 master weights
+    const double* pos;      // [B][2N] configurations
+    const int* electrons;   // [B] moved electron
+    const double* normals;  // [B][2]
+    double* xnew;           // [B][2] proposed position of the electron
+    int* nnb;               // [B] neighbours in the features at x'
+    float* act;             // [B][2][f | h1 | h2 | o] activations of both passes
+    float* delta;           // [B][2][d1 | d2 | d3] layer deltas of both passes
+    const double* eval;     // [B][9 + 2N] the walk's evaluation after the move
+    double* stats;          // [B][5]: J, A, log|psi'/psi|^2, x', y'
+};
+
 // per-walker outputs of the parity kernel
 struct EvalOut {
+    const double* newv2;   // [W][2]: force a move of particle moves[w] to these coordinates (or null)
     double* o0;        // [W][9 + nd]
     double* o1;
     double* ratios;    // [W][num_elements] per-element ratio of the forced move, or null

@@ -23,6 +23,16 @@ hipError_t launch_eval(int precision, const KernelShape& sh, const WalkParams& P
                        hipStream_t stream);
 hipError_t launch_reduce_rows(const double* rows, int nrows, int width, double* out,
                               hipStream_t stream);
+hipError_t launch_lp_draws(const LpTrainParams& T, uint64_t iteration, int* electrons,
+                           double* normals, hipStream_t stream);
+hipError_t launch_lp_propose(const LpTrainParams& T, hipStream_t stream);
+hipError_t launch_lp_backward(const LpTrainParams& T, hipStream_t stream);
+hipError_t launch_lp_wgrad(const LpTrainParams& T, double* rows, hipStream_t stream);
+hipError_t launch_lp_adam(float* theta, uint16_t* bf16, float* m, float* v, const double* grad,
+                          double inv_b, double lr, int step, hipStream_t stream);
+int lp_wgrad_rows(int B);
+int lp_act_floats();
+int lp_delta_floats();
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
                                  hipStream_t stream);
@@ -94,6 +104,34 @@ struct vmc_engine {
     double* d_rbm_acc = nullptr;   // [blocks][row width]
     double* d_rbm_out = nullptr;   // [row width]
     int rbm_blocks = 0, rbm_width = 0;
+    // learned proposal
+    bool lp_set = false;
+    vmc_lp_hyper lp_hp{0.2, 1.0, 2.0};
+    float* d_lp_theta = nullptr;   // FP32 master weights
+    uint16_t* d_lp_bf16 = nullptr; // bfloat16 copy used by the walk
+    float* d_lp_m = nullptr;       // ADAM moments
+    float* d_lp_v = nullptr;
+    int lp_step = 0;
+    uint64_t lp_iter = 0;
+    // training buffers for one sample per walker
+    int lp_B = 0;
+    int* d_lp_el = nullptr;
+    int* d_lp_nnb = nullptr;
+    double *d_lp_nrm = nullptr, *d_lp_xnew = nullptr, *d_lp_o0 = nullptr, *d_lp_o1 = nullptr;
+    double *d_lp_stats = nullptr, *d_lp_rows = nullptr, *d_lp_grad = nullptr;
+    float *d_lp_act = nullptr, *d_lp_delta = nullptr;
+
+    void free_lp_train()
+    {
+        for (void* b : {(void*)d_lp_el, (void*)d_lp_nnb, (void*)d_lp_nrm, (void*)d_lp_xnew,
+                        (void*)d_lp_o0, (void*)d_lp_o1, (void*)d_lp_stats, (void*)d_lp_rows,
+                        (void*)d_lp_grad, (void*)d_lp_act, (void*)d_lp_delta})
+            if (b) (void)hipFree(b);
+        d_lp_el = d_lp_nnb = nullptr;
+        d_lp_nrm = d_lp_xnew = d_lp_o0 = d_lp_o1 = d_lp_stats = d_lp_rows = d_lp_grad = nullptr;
+        d_lp_act = d_lp_delta = nullptr;
+        lp_B = 0;
+    }
     double alpha = 1.0, gamma = 1.0;
     bool params_set = false;
     bool walkers_set = false;
@@ -119,6 +157,9 @@ struct vmc_engine {
         if (d_rec) (void)hipFree(d_rec);
         for (double* b : {d_rbm_a, d_rbm_b, d_rbm_W, d_rbm_acc, d_rbm_out})
             if (b) (void)hipFree(b);
+        for (void* b : {(void*)d_lp_theta, (void*)d_lp_bf16, (void*)d_lp_m, (void*)d_lp_v})
+            if (b) (void)hipFree(b);
+        free_lp_train();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -179,7 +220,7 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
         for (int f = 0; f < e; f++)
             if (c.kinds[f] == k) return fail(VMC_E_INVALID, "an element appears twice");
     }
-    if (c.sampling < VMC_BRUTE_FORCE || c.sampling > VMC_IMPORTANCE_EXACT)
+    if (c.sampling < VMC_BRUTE_FORCE || c.sampling > VMC_LEARNED)
         return fail(VMC_E_INVALID, "unknown sampling method");
     if (c.precision != 32 && c.precision != 64)
         return fail(VMC_E_INVALID, "precision must be 32 or 64");
@@ -190,6 +231,8 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
     if (!(c.drift_limit >= 0)) return fail(VMC_E_INVALID, "drift_limit must be >= 0");
 
     const bool rbm = has(&c, VMC_RBM_GAUSSIAN) || has(&c, VMC_RBM_PRODUCT);
+    if (rbm && c.sampling == VMC_LEARNED)
+        return fail(VMC_E_INVALID, "the learned proposal is not available with RBM elements");
     if (rbm) {
         if (2 * c.num_particles > kRbmMaxDof)
             return fail(VMC_E_INVALID, "RBM elements: numParticles * numDimensions must be <= 32");
@@ -403,6 +446,11 @@ static vmc::WalkParams base_params(const vmc_engine* e)
     P.rbm_b = e->d_rbm_b;
     P.rbm_W = e->d_rbm_W;
     P.rbm_acc = e->d_rbm_acc;
+    P.lp_w = e->d_lp_bf16;
+    P.lp_sigma0 = e->lp_hp.sigma0;
+    P.lp_mu_max = e->lp_hp.mu_max;
+    P.lp_ls_max = e->lp_hp.ls_max;
+    if (c.sampling == VMC_LEARNED) P.q2_upper = 0;   // the exact |psi'/psi|^2
     P.seed = e->seed;
     P.W = c.num_walkers;
     return P;
@@ -416,6 +464,8 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
     if (!e->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
     if (steps < 1 || equil < 0 || batches < 1 || iteration < 0)
         return fail(VMC_E_INVALID, "steps >= 1, equilibration >= 0, batches >= 1 required");
+    if (e->cfg.sampling == VMC_LEARNED && !e->lp_set)
+        return fail(VMC_E_STATE, "the learned proposal has no weights (vmc_lp_set)");
     if (rec_walkers < 0 || rec_walkers > e->cfg.num_walkers)
         return fail(VMC_E_INVALID, "record_walkers out of range");
     const int W = e->cfg.num_walkers;
@@ -625,6 +675,204 @@ int vmc_evaluate_elements(vmc_engine* e, const double* positions, int walkers, c
     if (!ratios || !grads_before || !grads_after) return fail(VMC_E_INVALID, "null argument");
     return evaluate_impl(e, positions, walkers, moves, new_values, out_before, out_after, ratios,
                          grads_before, grads_after);
+}
+
+static uint16_t to_bf16(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);   // round to nearest even
+}
+
+int vmc_lp_set(vmc_engine* e, const float* theta, int n, const vmc_lp_hyper* hp)
+{
+    if (!e || !theta) return fail(VMC_E_INVALID, "null argument");
+    if (n != vmc::kLpParams) return fail(VMC_E_INVALID, "the network has VMC_LP_NPARAM parameters");
+    if (hp) {
+        if (!(hp->sigma0 > 0) || !(hp->mu_max > 0) || !(hp->ls_max > 0))
+            return fail(VMC_E_INVALID, "vmc_lp_hyper fields must be positive");
+        e->lp_hp = *hp;
+    }
+    const size_t n4 = sizeof(float) * vmc::kLpParams;
+    if (!e->d_lp_theta) {
+        HIP_TRY(hipMalloc(&e->d_lp_theta, n4));
+        HIP_TRY(hipMalloc(&e->d_lp_bf16, sizeof(uint16_t) * vmc::kLpParams));
+        HIP_TRY(hipMalloc(&e->d_lp_m, n4));
+        HIP_TRY(hipMalloc(&e->d_lp_v, n4));
+    }
+    std::vector<uint16_t> b(vmc::kLpParams);
+    for (int i = 0; i < vmc::kLpParams; i++) b[i] = to_bf16(theta[i]);
+    HIP_TRY(hipMemcpy(e->d_lp_theta, theta, n4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_lp_bf16, b.data(), sizeof(uint16_t) * vmc::kLpParams,
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(e->d_lp_m, 0, n4));
+    HIP_TRY(hipMemset(e->d_lp_v, 0, n4));
+    e->lp_step = 0;
+    e->lp_set = true;
+    return VMC_OK;
+}
+
+int vmc_lp_get(vmc_engine* e, float* theta, int n)
+{
+    if (!e || !theta) return fail(VMC_E_INVALID, "null argument");
+    if (n != vmc::kLpParams) return fail(VMC_E_INVALID, "the network has VMC_LP_NPARAM parameters");
+    if (!e->lp_set) return fail(VMC_E_STATE, "vmc_lp_set has not been called");
+    HIP_TRY(hipMemcpy(theta, e->d_lp_theta, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return VMC_OK;
+}
+
+// training buffers for B samples
+static int lp_buffers(vmc_engine* e, int B)
+{
+    if (e->lp_B == B) return VMC_OK;
+    e->free_lp_train();
+    const size_t w9 = (size_t)B * (9 + e->nd);
+    HIP_TRY(hipMalloc(&e->d_lp_el, sizeof(int) * B));
+    HIP_TRY(hipMalloc(&e->d_lp_nnb, sizeof(int) * B));
+    HIP_TRY(hipMalloc(&e->d_lp_nrm, sizeof(double) * 2 * B));
+    HIP_TRY(hipMalloc(&e->d_lp_xnew, sizeof(double) * 2 * B));
+    HIP_TRY(hipMalloc(&e->d_lp_o0, sizeof(double) * w9));
+    HIP_TRY(hipMalloc(&e->d_lp_o1, sizeof(double) * w9));
+    HIP_TRY(hipMalloc(&e->d_lp_stats, sizeof(double) * 5 * B));
+    HIP_TRY(hipMalloc(&e->d_lp_rows, sizeof(double) * (size_t)vmc::lp_wgrad_rows(B) * vmc::kLpParams));
+    HIP_TRY(hipMalloc(&e->d_lp_grad, sizeof(double) * vmc::kLpParams));
+    HIP_TRY(hipMalloc(&e->d_lp_act, sizeof(float) * (size_t)B * vmc::lp_act_floats()));
+    HIP_TRY(hipMalloc(&e->d_lp_delta, sizeof(float) * (size_t)B * vmc::lp_delta_floats()));
+    e->lp_B = B;
+    return VMC_OK;
+}
+
+// the training pipeline on B configurations at pos (device); electrons and
+// normals already in the training buffers; leaves the summed gradient in d_lp_grad
+static int lp_pipeline(vmc_engine* e, const double* pos, int B)
+{
+    vmc::LpTrainParams T{};
+    T.B = B;
+    T.N = e->cfg.num_particles;
+    T.omega = e->cfg.omega;
+    T.seed = e->seed;
+    T.sigma0 = e->lp_hp.sigma0;
+    T.mu_max = e->lp_hp.mu_max;
+    T.ls_max = e->lp_hp.ls_max;
+    T.theta = e->d_lp_theta;
+    T.pos = pos;
+    T.electrons = e->d_lp_el;
+    T.normals = e->d_lp_nrm;
+    T.xnew = e->d_lp_xnew;
+    T.nnb = e->d_lp_nnb;
+    T.act = e->d_lp_act;
+    T.delta = e->d_lp_delta;
+    T.eval = e->d_lp_o1;
+    T.stats = e->d_lp_stats;
+    HIP_TRY(vmc::launch_lp_propose(T, e->stream));
+    // |psi(x')/psi(x)|^2 and the drift at x': the walk's own evaluation of the
+    // whole-electron move (FP64 or FP32 as the engine), exact Pade ratio
+    vmc::WalkParams P = base_params(e);
+    P.W = B;
+    P.q2_upper = 0;
+    vmc::EvalOut O{};
+    O.newv2 = e->d_lp_xnew;
+    O.o0 = e->d_lp_o0;
+    O.o1 = e->d_lp_o1;
+    O.num_elements = e->cfg.num_elements;
+    O.maxpar = e->maxpar;
+    for (int i = 0; i < e->cfg.num_elements; i++) O.rows[i] = e->cfg.kinds[i];
+    HIP_TRY(vmc::launch_eval(e->cfg.precision, e->shape, P, pos, e->d_lp_el, e->d_lp_xnew, O,
+                             e->stream));
+    HIP_TRY(vmc::launch_lp_backward(T, e->stream));
+    HIP_TRY(vmc::launch_lp_wgrad(T, e->d_lp_rows, e->stream));
+    HIP_TRY(vmc::launch_reduce_rows(e->d_lp_rows, vmc::lp_wgrad_rows(B), vmc::kLpParams,
+                                    e->d_lp_grad, e->stream));
+    return VMC_OK;
+}
+
+static bool lp_usable(const vmc_engine* e) { return !e->shape.rbm; }
+
+int vmc_lp_gradient(vmc_engine* e, const double* positions, int walkers, const int* electrons,
+                    const double* normals, double* per_sample, double* grad)
+{
+    if (!e || !positions || !electrons || !normals || !per_sample || !grad)
+        return fail(VMC_E_INVALID, "null argument");
+    if (!e->lp_set) return fail(VMC_E_STATE, "vmc_lp_set has not been called");
+    if (!e->params_set) return fail(VMC_E_STATE, "vmc_set_parameters has not been called");
+    if (!lp_usable(e)) return fail(VMC_E_INVALID, "the learned proposal is not available with RBM elements");
+    if (walkers < 1) return fail(VMC_E_INVALID, "walkers must be positive");
+    for (int w = 0; w < walkers; w++)
+        if (electrons[w] < 0 || electrons[w] >= e->cfg.num_particles)
+            return fail(VMC_E_INVALID, "electron index out of range");
+    int rc = lp_buffers(e, walkers);
+    if (rc != VMC_OK) return rc;
+    double* d_pos = nullptr;
+    HIP_TRY(hipMalloc(&d_pos, sizeof(double) * walkers * e->nd));
+    hipError_t err;
+    if ((err = hipMemcpyAsync(d_pos, positions, sizeof(double) * walkers * e->nd,
+                              hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(e->d_lp_el, electrons, sizeof(int) * walkers, hipMemcpyHostToDevice,
+                              e->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(e->d_lp_nrm, normals, sizeof(double) * 2 * walkers,
+                              hipMemcpyHostToDevice, e->stream)) != hipSuccess) {
+        (void)hipFree(d_pos);
+        return fail(VMC_E_DEVICE, std::string("lp_gradient: ") + hipGetErrorString(err));
+    }
+    rc = lp_pipeline(e, d_pos, walkers);
+    if (rc == VMC_OK) {
+        if ((err = hipMemcpyAsync(per_sample, e->d_lp_stats, sizeof(double) * 5 * walkers,
+                                  hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(grad, e->d_lp_grad, sizeof(double) * vmc::kLpParams,
+                                  hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
+            (err = hipStreamSynchronize(e->stream)) != hipSuccess)
+            rc = fail(VMC_E_DEVICE, std::string("lp_gradient: ") + hipGetErrorString(err));
+    }
+    (void)hipFree(d_pos);
+    return rc;
+}
+
+int vmc_lp_train(vmc_engine* e, int iterations, int walk_steps, double learning_rate,
+                 double* history)
+{
+    if (!e) return fail(VMC_E_INVALID, "null engine");
+    if (!e->lp_set) return fail(VMC_E_STATE, "vmc_lp_set has not been called");
+    if (!e->params_set) return fail(VMC_E_STATE, "vmc_set_parameters has not been called");
+    if (!e->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
+    if (!lp_usable(e)) return fail(VMC_E_INVALID, "the learned proposal is not available with RBM elements");
+    if (iterations < 0 || walk_steps < 0 || !(learning_rate > 0))
+        return fail(VMC_E_INVALID, "iterations >= 0, walk_steps >= 0, learning_rate > 0 required");
+    const int B = e->cfg.num_walkers;
+    int rc = lp_buffers(e, B);
+    if (rc != VMC_OK) return rc;
+    std::vector<double> st(history ? (size_t)5 * B : 0);
+    for (int it = 0; it < iterations; it++) {
+        vmc::LpTrainParams T{};
+        T.B = B;
+        T.N = e->cfg.num_particles;
+        T.seed = e->seed;
+        HIP_TRY(vmc::launch_lp_draws(T, e->lp_iter++, e->d_lp_el, e->d_lp_nrm, e->stream));
+        rc = lp_pipeline(e, e->d_pos, B);
+        if (rc != VMC_OK) return rc;
+        e->lp_step += 1;
+        HIP_TRY(vmc::launch_lp_adam(e->d_lp_theta, e->d_lp_bf16, e->d_lp_m, e->d_lp_v, e->d_lp_grad,
+                                    1.0 / B, learning_rate, e->lp_step, e->stream));
+        if (history) {
+            HIP_TRY(hipMemcpyAsync(st.data(), e->d_lp_stats, sizeof(double) * 5 * B,
+                                   hipMemcpyDeviceToHost, e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+            double sJ = 0, sA = 0, sD = 0;
+            for (int b = 0; b < B; b++) {
+                sJ += st[5 * b];
+                sA += st[5 * b + 1];
+                sD += st[5 * b + 1] > 0 ? st[5 * b] / st[5 * b + 1] : 0.0;
+            }
+            history[3 * it] = sJ / B;
+            history[3 * it + 1] = sA / B;
+            history[3 * it + 2] = sD / B;
+        }
+        if (walk_steps > 0) {
+            rc = run_impl(e, walk_steps, 0, 0, 1, nullptr, nullptr, nullptr, 0, nullptr);
+            if (rc != VMC_OK) return rc;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return VMC_OK;
 }
 
 int vmc_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int n, int dof,

@@ -137,16 +137,20 @@ __device__ __forceinline__ R u23(uint32_t u)
     return (R(u >> 9) + R(0.5)) * R(1.0 / 8388608.0);
 }
 
-// step draws: coordinate, standard normal (Box-Muller), move uniform, accept uniform
+// step draws: coordinate, standard normal (Box-Muller), move uniform, accept uniform,
+// and the Box-Muller partner normal z2 (used by particle moves)
 template <typename R>
 __device__ __forceinline__ void step_draws(uint64_t seed, uint32_t walker, uint64_t counter,
-                                           int nd, int& k, R& z, R& um, R& ua)
+                                           int nd, int& k, R& z, R& um, R& ua, R& z2,
+                                           uint32_t stream = 0u)
 {
-    uint32_t c[4] = {(uint32_t)counter, (uint32_t)(counter >> 32), walker, 0u};
+    uint32_t c[4] = {(uint32_t)counter, (uint32_t)(counter >> 32), walker, stream};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     k = (int)__umulhi(c[0], (uint32_t)nd);
     const R U1 = u23<R>(c[1]), U2 = u23<R>(c[2]);
-    z = M<R>::sqrt(R(-2) * M<R>::log(U1)) * M<R>::cos2pi(U2);
+    const R r = M<R>::sqrt(R(-2) * M<R>::log(U1));
+    z = r * M<R>::cos2pi(U2);
+    z2 = r * M<R>::cos2pi(U2 - R(0.25));   // sin(2 pi U2)
     um = U1;
     ua = u23<R>(c[3]);
 }
@@ -658,12 +662,23 @@ struct StepInfo {
     R rrg, rrp;      // RBM-Gaussian and RBM-product ratios
 };
 
+// a whole-particle move (learned proposal): particle p to (x0, x1), accepted if
+// log|psi'/psi|^2 + logq >= log u (logq = log q(x|x') - log q(x'|x))
+template <typename R>
+struct ParticleMove {
+    int p;
+    R x0, x1;
+    R logq;
+};
+
 template <typename R, int NP, int K, bool RB>
 __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkParams& P, int k,
                                             R z, R um, R ua, bool forced, R xforced,
-                                            StepInfo<R>& info)
+                                            StepInfo<R>& info,
+                                            const ParticleMove<R>* pm = nullptr)
 {
     using WK = Walker<R, NP, K, RB>;
+    if (pm) k = 2 * pm->p;
     const R dt = R(P.dt), sqdt = R(::sqrt(P.dt));
     const R oa = R(P.omega * P.alpha);
     const R gam = R(P.gamma);
@@ -707,8 +722,8 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
     else if (P.sampling == kBruteForce) dx = R(10) * (um - R(0.5)) * dt;
     else dx = R(0.5) * dt * Fp + sqdt * z;
     const R xnd = forced ? xforced : xpd + dx;
-    const R xn0 = d ? xp0 : xnd;
-    const R xn1 = d ? xnd : xp1;
+    const R xn0 = pm ? pm->x0 : (d ? xp0 : xnd);
+    const R xn1 = pm ? pm->x1 : (d ? xnd : xp1);
 
     // pair terms of the moved particle against the owned ones
     R jexp = R(0), dvi = R(0), dlj = R(0), djg = R(0), gp0 = R(0), gp1 = R(0);
@@ -857,13 +872,29 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
     // acceptance (ref importancesampling.cpp:43-47, 68-81; bruteforce.cpp:42-52)
     info.rs2 = Rs * Rs;
     info.rp = P.has_pade ? M<R>::exp(R(2) * jexp) : R(1);
-    info.rg = P.has_gauss ? M<R>::exp(oa * (xpd * xpd - xnd * xnd)) : R(1);
+    info.rg = P.has_gauss ? M<R>::exp(pm ? oa * (xp0 * xp0 + xp1 * xp1 - xn0 * xn0 - xn1 * xn1)
+                                         : oa * (xpd * xpd - xnd * xnd))
+                          : R(1);
     info.rrg = rrg;
     info.rrp = rrp;
     const R ratio = info.rs2 * info.rp * info.rg * rrg * rrp;
+    // log of the ratio as a sum of logs: whole-electron moves change the Pade
+    // exponent by tens, and the product of the factors would leave the FP32 range
+    auto log_ratio = [&]() {
+        R lr = R(0);
+        if (P.has_slater) lr += R(2) * M<R>::log(Rs < R(0) ? -Rs : Rs);
+        if (P.has_pade) lr += R(2) * jexp;
+        if (P.has_gauss)
+            lr += pm ? oa * (xp0 * xp0 + xp1 * xp1 - xn0 * xn0 - xn1 * xn1)
+                     : oa * (xpd * xpd - xnd * xnd);
+        if constexpr (RB) lr += M<R>::log(rrg) + M<R>::log(rrp);
+        return lr;
+    };
     bool accept;
     if (forced) {
         accept = true;
+    } else if (pm) {
+        accept = log_ratio() + pm->logq >= M<R>::log(ua);
     } else if (P.sampling == kBruteForce) {
         accept = !(ratio < ua);
     } else if (P.sampling == kImportanceCompat) {
@@ -876,7 +907,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
             // FP32: the same test in the log domain, log w = softplus(xg) + log ratio,
             // which neither overflows nor forms inf x 0 next to a node
             const R sp = (xg > R(0) ? xg : R(0)) + M<R>::log(R(1) + M<R>::exp(-(xg > R(0) ? xg : -xg)));
-            accept = sp + M<R>::log(ratio) >= M<R>::log(ua);
+            accept = sp + log_ratio() >= M<R>::log(ua);
         }
     } else {
         R Gn = R(0);
@@ -896,7 +927,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         if constexpr (std::is_same<R, double>::value)
             accept = ratio * M<R>::exp(lg) >= ua;
         else
-            accept = M<R>::log(ratio) + lg >= M<R>::log(ua);
+            accept = log_ratio() + lg >= M<R>::log(ua);
     }
     if (!accept) return false;
 
@@ -904,7 +935,10 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         const bool me = owner && s == slot;
-        if (me) {
+        if (me && pm) {
+            w.x[s][0] = xn0;
+            w.x[s][1] = xn1;
+        } else if (me) {
             if (d) w.x[s][1] = xnd; else w.x[s][0] = xnd;
             if (P.sampling == kImportanceCompat) {
                 if (d) w.qf[s][1] = F; else w.qf[s][0] = F;
@@ -915,7 +949,11 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
             w.gJ[s][1] = me ? gp1 : w.gJ[s][1] + (w.valid[s] ? dg[s][1] : R(0));
         }
     }
-    w.sumx2 += double(xnd) * double(xnd) - double(xpd) * double(xpd);
+    if (pm)
+        w.sumx2 += double(xn0) * double(xn0) + double(xn1) * double(xn1) - double(xp0) * double(xp0) -
+                   double(xp1) * double(xp1);
+    else
+        w.sumx2 += double(xnd) * double(xnd) - double(xpd) * double(xpd);
     // only group-reduced increments may enter the walker sums
     if (P.coulomb) w.vint += double(dvi);
     if (P.has_pade) {
@@ -987,6 +1025,211 @@ __device__ __forceinline__ R walker_gradient(const Walker<R, NP, K, RB>& w, cons
 }
 
 // ------------------------------------------------------------------------
+// learned proposal: a per-electron MLP evaluated by the walker's K lanes.
+// Features of electron i: its coordinates and the relative coordinates of its
+// 8 nearest electrons, in units of 1/sqrt(omega); two tanh layers of 64 (lane
+// l computes units l + K j); 4 outputs: move mean and log-scale.  Weights are
+// bfloat16 in LDS, arithmetic in the walk precision R.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_float(uint16_t b)
+{
+    return __uint_as_float((uint32_t)b << 16);
+}
+
+template <typename R>
+__device__ __forceinline__ R lp_tanh(R a)
+{
+    if constexpr (std::is_same<R, double>::value) {
+        return ::tanh(a);
+    } else {
+        const R e = M<R>::exp(R(2) * a);
+        return R(1) - R(2) * M<R>::rcp(e + R(1));
+    }
+}
+
+template <typename R, int K>
+struct LpLds {
+    uint16_t* W;   // [kLpParams]
+    R* h;          // [G][kLpHid] first-layer activations of each walker
+    __device__ __forceinline__ R w(int i) const { return R(bf16_to_float(W[i])); }
+};
+
+template <typename R, int K>
+__device__ __forceinline__ void lp_carve(LpLds<R, K>& L, double* base)
+{
+    L.h = reinterpret_cast<R*>(base);
+    L.W = reinterpret_cast<uint16_t*>(L.h + (64 / K) * kLpHid);
+}
+
+inline size_t lp_lds_bytes(int K, size_t rsize)
+{
+    return rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLpParams;
+}
+
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ void lp_features(const Walker<R, NP, K, RB>& w, const WalkParams& P, int i,
+                                            R xi, R yi, R (&f)[kLpIn])
+{
+    const R s = R(::sqrt(P.omega));
+    f[0] = s * xi;
+    f[1] = s * yi;
+    bool taken0 = false, taken1 = false;
+    constexpr int kNone = 1 << 30;
+    sfor<0, 8>([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        // nearest remaining electron: squared distance, ties to the lower index
+        R bv = R(0);
+        int bi = kNone;
+#pragma unroll
+        for (int sl = 0; sl < 2; sl++) {
+            const int q = sl * NP + w.l;
+            const bool tk = sl ? taken1 : taken0;
+            const R e0 = w.x[sl][0] - xi, e1 = w.x[sl][1] - yi;
+            const R d2 = e0 * e0 + e1 * e1;
+            const bool ok = w.valid[sl] && q != i && !tk;
+            if (ok && (bi == kNone || d2 < bv || (d2 == bv && q < bi))) {
+                bv = d2;
+                bi = q;
+            }
+        }
+#pragma unroll
+        for (int msk = K / 2; msk >= 1; msk >>= 1) {
+            const R ov = __shfl_xor(bv, msk, 64);
+            const int oi = __shfl_xor(bi, msk, 64);
+            if (oi != kNone && (bi == kNone || ov < bv || (ov == bv && oi < bi))) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        const bool found = bi != kNone;
+        const int j = found ? bi : 0;
+        const int js = j / NP, jl = j - js * NP;
+        const R xj0 = __shfl(vsel(js != 0, w.x[1][0], w.x[0][0]), w.gbase + jl, 64);
+        const R xj1 = __shfl(vsel(js != 0, w.x[1][1], w.x[0][1]), w.gbase + jl, 64);
+        f[2 + 2 * m] = found ? s * (xj0 - xi) : R(0);
+        f[3 + 2 * m] = found ? s * (xj1 - yi) : R(0);
+        if (found && w.l == jl) {
+            if (js) taken1 = true;
+            else taken0 = true;
+        }
+    });
+}
+
+// network outputs o[4] of features f (all K lanes of the walker take part).
+// Weight rows are read from LDS 8 bfloat16 at a time (W1 rows 2 at a time).
+__device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xFFFF0000u); }
+
+template <typename R, int K>
+__device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const R (&f)[kLpIn],
+                                       R (&o)[kLpOut])
+{
+    constexpr int UPL = kLpHid / K;
+    constexpr int UNR = UPL <= 4 ? UPL : 1;   // unroll only the few-units-per-lane shapes
+    R* hb = L.h + g * kLpHid;
+    const uint32_t* W32 = reinterpret_cast<const uint32_t*>(L.W);
+#pragma unroll UNR
+    for (int j = 0; j < UPL; j++) {
+        const int u = l + K * j;
+        R a = L.w(kLpB1 + u);
+        const uint32_t* row = W32 + (kLpW1 + u * kLpIn) / 2;   // 18 weights = 9 words, 4-byte aligned
+#pragma unroll
+        for (int m = 0; m < kLpIn / 2; m++) {
+            const uint32_t v = row[m];
+            a += R(bf16_lo(v)) * f[2 * m];
+            a += R(bf16_hi(v)) * f[2 * m + 1];
+        }
+        hb[u] = lp_tanh(a);
+    }
+    __syncthreads();
+    R t[kLpOut] = {R(0), R(0), R(0), R(0)};
+    if constexpr (UPL <= 4) {
+        // c-blocks of 8: the activations once, then each unit's 8 weights in one read
+        R a2[UPL];
+#pragma unroll
+        for (int j = 0; j < UPL; j++) a2[j] = L.w(kLpB2 + l + K * j);
+        for (int c = 0; c < kLpHid; c += 8) {
+            R h[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) h[q] = hb[c + q];
+#pragma unroll
+            for (int j = 0; j < UPL; j++) {
+                const int u = l + K * j;
+                const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLpW2 + u * kLpHid + c);
+                a2[j] += R(bf16_lo(v.x)) * h[0] + R(bf16_hi(v.x)) * h[1] + R(bf16_lo(v.y)) * h[2] +
+                         R(bf16_hi(v.y)) * h[3] + R(bf16_lo(v.z)) * h[4] + R(bf16_hi(v.z)) * h[5] +
+                         R(bf16_lo(v.w)) * h[6] + R(bf16_hi(v.w)) * h[7];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < UPL; j++) {
+            const R h2 = lp_tanh(a2[j]);
+#pragma unroll
+            for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLpW3 + r * kLpHid + l + K * j) * h2;
+        }
+    } else {
+        // many units per lane: one unit at a time, the head folded in
+        for (int j = 0; j < UPL; j++) {
+            const int u = l + K * j;
+            R a2 = L.w(kLpB2 + u);
+            for (int c = 0; c < kLpHid; c += 8) {
+                const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLpW2 + u * kLpHid + c);
+                a2 += R(bf16_lo(v.x)) * hb[c] + R(bf16_hi(v.x)) * hb[c + 1] +
+                      R(bf16_lo(v.y)) * hb[c + 2] + R(bf16_hi(v.y)) * hb[c + 3] +
+                      R(bf16_lo(v.z)) * hb[c + 4] + R(bf16_hi(v.z)) * hb[c + 5] +
+                      R(bf16_lo(v.w)) * hb[c + 6] + R(bf16_hi(v.w)) * hb[c + 7];
+            }
+            const R h2 = lp_tanh(a2);
+#pragma unroll
+            for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLpW3 + r * kLpHid + u) * h2;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLpB3 + r);
+}
+
+// move mean and scale: mu = l_mu tanh(o/l_mu) / sqrt(omega), scale =
+// sigma0 exp(ls tanh(o/ls)) / sqrt(omega)
+template <typename R>
+__device__ __forceinline__ void lp_head(const WalkParams& P, const R (&o)[kLpOut], R (&mu)[2],
+                                        R (&sg)[2])
+{
+    const R ell = R(1.0 / ::sqrt(P.omega));
+    const R mm = R(P.lp_mu_max), ls = R(P.lp_ls_max);
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        mu[d] = ell * mm * lp_tanh(o[d] / mm);
+        sg[d] = R(P.lp_sigma0) * ell * M<R>::exp(ls * lp_tanh(o[2 + d] / ls));
+    }
+}
+
+// the learned proposal of electron i with normals z0, z1: the move and
+// log q(x|x') - log q(x'|x)
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ ParticleMove<R> lp_propose(const Walker<R, NP, K, RB>& w,
+                                                      const WalkParams& P, const LpLds<R, K>& L,
+                                                      int i, R z0, R z1)
+{
+    const int slot = i >= NP ? 1 : 0, li = i - slot * NP;
+    const R xi = __shfl(vsel(slot != 0, w.x[1][0], w.x[0][0]), w.gbase + li, 64);
+    const R yi = __shfl(vsel(slot != 0, w.x[1][1], w.x[0][1]), w.gbase + li, 64);
+    R f[kLpIn], o[kLpOut], mu[2], sg[2];
+    lp_features(w, P, i, xi, yi, f);
+    lp_mlp(L, w.g, w.l, f, o);
+    lp_head(P, o, mu, sg);
+    const R xn = xi + mu[0] + sg[0] * z0, yn = yi + mu[1] + sg[1] * z1;
+    const R lqf = -(R(0.5) * z0 * z0 + M<R>::log(sg[0])) - (R(0.5) * z1 * z1 + M<R>::log(sg[1]));
+    lp_features(w, P, i, xn, yn, f);
+    lp_mlp(L, w.g, w.l, f, o);
+    lp_head(P, o, mu, sg);
+    const R r0 = xi - xn - mu[0], r1 = yi - yn - mu[1];
+    const R lqr = -(r0 * r0 / (R(2) * sg[0] * sg[0]) + M<R>::log(sg[0])) -
+                  (r1 * r1 / (R(2) * sg[1] * sg[1]) + M<R>::log(sg[1]));
+    return ParticleMove<R>{i, xn, yn, lqr - lqf};
+}
+
+// ------------------------------------------------------------------------
 // RBM block-level LDS: gradient-window sums of the wave, then W and the
 // walkers' sorted coordinate vectors and n.  Sized at launch (dynamic LDS).
 // ------------------------------------------------------------------------
@@ -1029,7 +1272,7 @@ __device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& 
 // ------------------------------------------------------------------------
 // the walk kernel: equil + steps Metropolis steps per walker
 // ------------------------------------------------------------------------
-template <typename R, int NP, int K, bool RB>
+template <typename R, int NP, int K, bool RB, bool LP>
 __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
 {
     using WK = Walker<R, NP, K, RB>;
@@ -1050,7 +1293,13 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
     const int glive = (P.W - (int)blockIdx.x * G) < G ? (P.W - (int)blockIdx.x * G) : G;
 
     RbmLds<R, K> L;
+    LpLds<R, K> LL;
     walker_load(w, P, P.pos + (size_t)walker * nd, P.qf ? P.qf + (size_t)walker * nd : nullptr);
+    if constexpr (LP) {
+        lp_carve(LL, rbm_dyn);
+        for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
+        __syncthreads();
+    }
     if constexpr (RB) {
         L.carve(rbm_dyn, P.ND, P.H);
         rbm_bind(w, L, P);
@@ -1081,13 +1330,13 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
 
     // random draws for K steps at a time, one step per lane of the group
     int rk = 0;
-    R rz = R(0), rum = R(0), rua = R(0);
+    R rz = R(0), rum = R(0), rua = R(0), rz2 = R(0);
 
     for (int t = P.t_begin; t < P.t_end; t++) {
         const int tk = t & (K - 1);
         if (tk == 0 || t == P.t_begin)
             step_draws<R>(P.seed, (uint32_t)walker, P.counter0 + (uint64_t)(t - tk + w.l), nd, rk,
-                          rz, rum, rua);
+                          rz, rum, rua, rz2);
         const int src = w.gbase + tk;
         const int k = __shfl(rk, src, 64);
         const R z = __shfl(rz, src, 64);
@@ -1095,7 +1344,15 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
         const R ua = __shfl(rua, src, 64);
 
         StepInfo<R> info;
-        const bool accept = walker_step(w, P, k, z, um, ua, false, R(0), info);
+        bool accept;
+        if constexpr (LP) {
+            // electron k/2, normals (z, z2) of the same Box-Muller pair
+            const R z2 = __shfl(rz2, src, 64);
+            const ParticleMove<R> mv = lp_propose(w, P, LL, k >> 1, z, z2);
+            accept = walker_step(w, P, k, z, um, ua, false, R(0), info, &mv);
+        } else {
+            accept = walker_step(w, P, k, z, um, ua, false, R(0), info);
+        }
         if constexpr (RB) w.rbm_after_step(P, accept);
 
         if (t >= P.equil) {
@@ -1318,7 +1575,13 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
     for (int pass = 0; pass < 2; pass++) {
         StepInfo<R> info{R(0), R(0), R(0), R(0), R(0)};
         if (pass == 1) {
-            walker_step(w, P, moves[walker], R(0), R(0), R(0), true, R(newv[walker]), info);
+            if (O.newv2) {
+                const ParticleMove<R> mv{moves[walker], R(O.newv2[2 * walker]),
+                                         R(O.newv2[2 * walker + 1]), R(0)};
+                walker_step(w, P, 0, R(0), R(0), R(0), true, R(0), info, &mv);
+            } else {
+                walker_step(w, P, moves[walker], R(0), R(0), R(0), true, R(newv[walker]), info);
+            }
             if constexpr (RB) w.rbm_after_step(P, true);
         }
         double* o = (pass ? O.o1 : O.o0) + (size_t)walker * width;
@@ -1369,13 +1632,14 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
 // ------------------------------------------------------------------------
 using RT = std::conditional_t<VMC_WALK_PRECISION == 32, float, double>;
 
-template <int NP, int K, bool RB>
+template <int NP, int K, bool RB, bool LP>
 static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
 {
     constexpr int G = 64 / K;
     const int blocks = (P.W + G - 1) / G;
-    const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT)) : 0;
-    hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB>), dim3(blocks), dim3(64), lds, stream, P);
+    const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT))
+                          : (LP ? lp_lds_bytes(K, sizeof(RT)) : 0);
+    hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP>), dim3(blocks), dim3(64), lds, stream, P);
     return hipGetLastError();
 }
 
@@ -1405,11 +1669,22 @@ static hipError_t launch_eval_t(const WalkParams& P, const double* pos, const in
     X(1, 1, true) X(1, 2, true) X(1, 4, true) X(1, 8, true) X(3, 4, true)    \
     X(3, 8, true) X(6, 8, true) X(10, 16, true)
 
-hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, const WalkParams& P,
+// shapes with the learned proposal (no RBM elements)
+#define VMC_LP_SHAPES(X) X(1, 1) X(3, 4) X(6, 8) X(10, 16) X(15, 16) X(21, 32) X(28, 32)
+
+hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, bool lp, const WalkParams& P,
                                    hipStream_t stream)
 {
+    if (lp) {
+        if (rbm) return hipErrorInvalidValue;
+#define Y(a, b) \
+    if (np == a && K == b) return launch_walk_t<a, b, false, true>(P, stream);
+        VMC_LP_SHAPES(Y)
+#undef Y
+        return hipErrorInvalidValue;
+    }
 #define X(a, b, c) \
-    if (np == a && K == b && rbm == c) return launch_walk_t<a, b, c>(P, stream);
+    if (np == a && K == b && rbm == c) return launch_walk_t<a, b, c, false>(P, stream);
     VMC_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -1434,8 +1709,8 @@ __global__ void draws_kernel(uint64_t seed, uint32_t walker, uint64_t counter0, 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int kk;
-    R zz, m, a;
-    step_draws<R>(seed, walker, counter0 + (uint64_t)i, nd, kk, zz, m, a);
+    R zz, m, a, z2;
+    step_draws<R>(seed, walker, counter0 + (uint64_t)i, nd, kk, zz, m, a, z2);
     k[i] = kk;
     z[i] = double(zz);
     um[i] = double(m);
@@ -1488,8 +1763,10 @@ __global__ void init_positions_kernel(double* pos, int W, int nd, double omega, 
     if (2 * blk + 1 < nd) pos[(size_t)w * nd + 2 * blk + 1] = r * ::sin(th) / omega;
 }
 
-hipError_t launch_walk_f32(int np, int K, bool rbm, const WalkParams& P, hipStream_t stream);
-hipError_t launch_walk_f64(int np, int K, bool rbm, const WalkParams& P, hipStream_t stream);
+hipError_t launch_walk_f32(int np, int K, bool rbm, bool lp, const WalkParams& P,
+                           hipStream_t stream);
+hipError_t launch_walk_f64(int np, int K, bool rbm, bool lp, const WalkParams& P,
+                           hipStream_t stream);
 hipError_t launch_eval_f32(int np, int K, bool rbm, const WalkParams& P, const double* pos,
                            const int* moves, const double* newv, const EvalOut& O,
                            hipStream_t stream);
@@ -1505,8 +1782,9 @@ hipError_t launch_draws_f64(uint64_t seed, uint32_t walker, uint64_t counter0, i
 hipError_t launch_walk(int precision, const KernelShape& sh, const WalkParams& P,
                        hipStream_t stream)
 {
-    return precision == 32 ? launch_walk_f32(sh.np, sh.K, sh.rbm, P, stream)
-                           : launch_walk_f64(sh.np, sh.K, sh.rbm, P, stream);
+    const bool lp = P.sampling == kLearned;
+    return precision == 32 ? launch_walk_f32(sh.np, sh.K, sh.rbm, lp, P, stream)
+                           : launch_walk_f64(sh.np, sh.K, sh.rbm, lp, P, stream);
 }
 
 hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int nd,

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(PKG, "_lib", "libvmc_hip.so")
 HEADER = os.path.join(ROOT, "include", "vmc.h")
 
 GAUSSIAN, SLATER, PADE, RBM_GAUSSIAN, RBM_PRODUCT = 1, 2, 3, 4, 5
-BRUTE_FORCE, IMPORTANCE, IMPORTANCE_EXACT = 0, 1, 2
+BRUTE_FORCE, IMPORTANCE, IMPORTANCE_EXACT, LEARNED = 0, 1, 2, 3
+LP_NPARAM = 5636
 Q2_PADE_RATIO_UPPER, Q3_PADE_GAMMA_HALF = 1 << 2, 1 << 3
 Q6_RBM_W_UNSCALED, Q13_RBMG_PSI_RATIO = 1 << 6, 1 << 13
 QUIRKS_REFERENCE = Q2_PADE_RATIO_UPPER | Q3_PADE_GAMMA_HALF | Q6_RBM_W_UNSCALED | Q13_RBMG_PSI_RATIO
@@ -48,6 +49,10 @@ class VmcSums(ctypes.Structure):
 
 class VmcError(RuntimeError):
     pass
+
+
+class LpHyper(ctypes.Structure):
+    _fields_ = [("sigma0", _D), ("mu_max", _D), ("ls_max", _D)]
 
 
 _lib = None
@@ -89,6 +94,10 @@ def lib():
                                             _DP, _DP]
         L.vmc_draws.argtypes = [_I, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _I, _I,
                                 _IP, _DP, _DP, _DP]
+        L.vmc_lp_set.argtypes = [ctypes.c_void_p, _FP, _I, ctypes.POINTER(LpHyper)]
+        L.vmc_lp_get.argtypes = [ctypes.c_void_p, _FP, _I]
+        L.vmc_lp_gradient.argtypes = [ctypes.c_void_p, _DP, _I, _IP, _DP, _DP, _DP]
+        L.vmc_lp_train.argtypes = [ctypes.c_void_p, _I, _I, _D, _DP]
         _lib = L
     return _lib
 
@@ -236,6 +245,35 @@ class Engine:
         self._check(self.L.vmc_evaluate(self.h, _dp(pos), W, mv.ctypes.data_as(_IP), _dp(nv),
                                         _dp(o0), _dp(o1)))
         return o0, o1
+
+    # ---- learned proposal ----
+    def lp_set(self, theta, sigma0=0.2, mu_max=1.0, ls_max=2.0):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        assert th.size == LP_NPARAM
+        hp = LpHyper(sigma0, mu_max, ls_max)
+        self._check(self.L.vmc_lp_set(self.h, th.ctypes.data_as(_FP), LP_NPARAM, ctypes.byref(hp)))
+
+    def lp_get(self):
+        th = np.zeros(LP_NPARAM, dtype=np.float32)
+        self._check(self.L.vmc_lp_get(self.h, th.ctypes.data_as(_FP), LP_NPARAM))
+        return th
+
+    def lp_gradient(self, positions, electrons, normals):
+        """Per-sample (J, A, log|psi'/psi|^2, x', y') and the summed dJ/dtheta."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        W = pos.shape[0]
+        el = np.ascontiguousarray(electrons, dtype=np.int32)
+        nr = np.ascontiguousarray(normals, dtype=np.float64).reshape(W, 2)
+        per = np.zeros((W, 5))
+        g = np.zeros(LP_NPARAM)
+        self._check(self.L.vmc_lp_gradient(self.h, _dp(pos), W, el.ctypes.data_as(_IP), _dp(nr),
+                                           _dp(per), _dp(g)))
+        return per, g
+
+    def lp_train(self, iterations, walk_steps, learning_rate):
+        hist = np.zeros((max(iterations, 1), 3))
+        self._check(self.L.vmc_lp_train(self.h, iterations, walk_steps, learning_rate, _dp(hist)))
+        return hist[:iterations]
 
     def evaluate_elements(self, positions, moves, new_values):
         """vmc_evaluate plus per-element ratios [W][E] of the forced move and the
