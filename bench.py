@@ -10,8 +10,10 @@ followed by the full local energy and dlnpsi/dtheta sample (the reference's
 per-step semantics, ref src/system.cpp:147-164, src/sampler.cpp:52-90).
 
 value = walker-steps/s of the FP32 walk (FP64 Slater-inverse rebuild every
-`--refresh` steps, FP64 accumulation).  The plain FP64 walk, the integrated
-autocorrelation time of E_L and the CPU baseline are reported beside it.
+`--refresh` steps, FP64 accumulation).  Beside it, all from the same
+equilibrated walkers: the plain FP64 walk, the learned whole-electron proposal
+(trained online on the engine's walkers), the integrated autocorrelation time
+of E_L of each and the IAT-corrected samples/s, and the CPU baseline.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2]
 """
@@ -195,6 +197,50 @@ def rbmpj_workload(args, V):
     print(json.dumps(out))
 
 
+def equilibrate(V, args):
+    """Walker positions at equilibrium: from N(0, 1/omega^2) per coordinate (ref
+    randomnormal.cpp:17), exact Langevin with a large step (dt=--equil-dt, drift
+    limit 1) for --equil-steps steps per walker.  Any exact method samples
+    |psi|^2, so the measured forms all start from these positions; at dt=0.005
+    Langevin itself would need ~10^6 steps per walker to get there."""
+    cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
+                        sampling=V.IMPORTANCE_EXACT, step_length=args.equil_dt, quirks=0,
+                        drift_limit=1.0, precision=32, refresh_interval=args.refresh,
+                        num_walkers=args.walkers)
+    with V.Engine(cfg) as eng:
+        eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
+        eng.init_walkers(args.seed)
+        burn_in(eng, args.equil_steps, f"equilibration (Langevin dt={args.equil_dt})")
+        return eng.positions()
+
+
+def lp_engine(V, args, pos):
+    cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
+                        sampling=V.LEARNED, step_length=args.dt, quirks=0, precision=32,
+                        refresh_interval=args.refresh, num_walkers=args.walkers)
+    eng = V.Engine(cfg)
+    eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
+    eng.lp_set(lp_initial_weights(args.seed))
+    eng.set_positions(pos)
+    eng.set_random_state(args.seed + 14, 0)
+    return eng
+
+
+def lp_initial_weights(seed):
+    """Glorot-uniform hidden layers, zero head: the untrained proposal is a
+    symmetric Gaussian random walk of width sigma0 = 0.2/sqrt(omega)."""
+    rng = np.random.default_rng(seed)
+    th = np.zeros(V_LP_NPARAM, dtype=np.float32)
+    a1, a2 = np.sqrt(6.0 / (18 + 64)), np.sqrt(6.0 / 128)
+    th[:64 * 18] = rng.uniform(-a1, a1, 64 * 18)
+    w2 = 64 * 18 + 64
+    th[w2:w2 + 64 * 64] = rng.uniform(-a2, a2, 64 * 64)
+    return th
+
+
+V_LP_NPARAM = 5636
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,16 +254,24 @@ def main():
     ap.add_argument("--refresh", type=int, default=100)
     ap.add_argument("--drift-limit", type=float, default=1.0,
                     help="Umrigar drift limit a of the exact Langevin proposal (0: none)")
-    ap.add_argument("--burnin", type=int, default=40000, help="burn-in steps per walker")
+    ap.add_argument("--burnin", type=int, default=40000,
+                    help="rbmpj workload: burn-in steps per walker")
+    ap.add_argument("--equil-steps", type=int, default=60000)
+    ap.add_argument("--equil-dt", type=float, default=0.1)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--iat-steps", type=int, default=8192)
-    ap.add_argument("--iat-walkers", type=int, default=512)
+    ap.add_argument("--iat-steps", type=int, default=32768)
+    ap.add_argument("--iat-walkers", type=int, default=256)
+    ap.add_argument("--lp-iat-steps", type=int, default=8192)
+    ap.add_argument("--lp-train-iters", type=int, default=300)
+    ap.add_argument("--lp-train-steps", type=int, default=10)
+    ap.add_argument("--lp-lr", type=float, default=3e-3)
     ap.add_argument("--fp64-iters", type=int, default=3)
     ap.add_argument("--cpu-steps-per-rank", type=int, default=200000)
     ap.add_argument("--cpu-port", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-iat", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--workload", choices=["dot56", "rbmpj"], default="dot56",
                     help="dot56: the headline N=56 workload; rbmpj: RBM x Slater x Pade N=12 H=24")
     args = ap.parse_args()
@@ -234,12 +288,16 @@ def main():
     if args.workload == "rbmpj":
         rbmpj_workload(args, V)
         return
-    n = args.N // 2 if args.N > 2 else 1
     n = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28}[args.N]
+    pos = equilibrate(V, args)
 
-    # ---- FP32 walk (value) ----
+    # ---- FP32 walk, exact Langevin dt (value) ----
+    # every measured form starts from the equilibrated positions with a random
+    # stream of its own (reusing the equilibration's stream would replay the
+    # noise that produced the starting state)
     eng = make_engine(V, args, 32, args.walkers)
-    burn_in(eng, args.burnin, "fp32")
+    eng.set_positions(pos)
+    eng.set_random_state(args.seed + 11, 0)
     wall, kern, E32, acc32 = timed(eng, args.steps, args.warmup, args.steps_per_iter)
     wsteps = args.walkers * args.steps_per_iter * args.steps
     value = wsteps / wall
@@ -253,14 +311,16 @@ def main():
     iat = None
     if not args.no_iat:
         tb, ts, Em, Ee = measure_iat(eng, args.iat_steps, args.iat_walkers)
-        iat = {"tau_int_blocking": tb, "tau_int_sokal": ts, "energy": Em, "energy_err": Ee}
+        iat = {"tau_int_blocking": tb, "tau_int_sokal": ts, "energy": Em, "energy_err": Ee,
+               "series_steps": args.iat_steps, "series_walkers": args.iat_walkers}
     eng.close()
 
-    # ---- plain FP64 walk ----
+    # ---- plain FP64 walk, exact Langevin dt ----
     fp64 = None
     if not args.no_fp64:
         eng = make_engine(V, args, 64, args.walkers)
-        burn_in(eng, args.burnin, "fp64")
+        eng.set_positions(pos)
+        eng.set_random_state(args.seed + 12, 0)
         w64, k64, E64, a64 = timed(eng, args.fp64_iters, 1, args.steps_per_iter)
         r64 = args.walkers * args.steps_per_iter * args.fp64_iters / w64
         fp64 = {"walker_steps_per_s": r64, "energy": E64, "acceptance": a64,
@@ -272,7 +332,49 @@ def main():
                          "iat_corrected_samples_per_s": r64 / (2 * tb)})
         eng.close()
 
+    # ---- learned proposal, FP32: online training, then the same measurements ----
+    learned = None
+    if not args.no_learned:
+        eng = lp_engine(V, args, pos)
+        t0 = time.perf_counter()
+        hist = eng.lp_train(args.lp_train_iters, args.lp_train_steps, args.lp_lr)
+        t_train = time.perf_counter() - t0
+        wl, kl, El, al = timed(eng, args.steps, args.warmup, args.steps_per_iter)
+        rl = wsteps / wl
+        learned = {"walker_steps_per_s": rl, "kernel_walker_steps_per_s": wsteps / kl,
+                   "energy": El, "acceptance": al,
+                   "training": {"iterations": args.lp_train_iters,
+                                "walk_steps_between": args.lp_train_steps,
+                                "learning_rate": args.lp_lr, "seconds": t_train,
+                                "esjd_start": float(hist[:5, 0].mean()),
+                                "esjd_end": float(hist[-5:, 0].mean()),
+                                "acceptance_end": float(hist[-5:, 1].mean())}}
+        if not args.no_iat:
+            tb, ts, Em, Ee = measure_iat(eng, args.lp_iat_steps, args.iat_walkers)
+            learned.update({"tau_int_blocking": tb, "tau_int_sokal": ts, "energy_iat_run": Em,
+                            "energy_err": Ee, "iat_corrected_samples_per_s": rl / (2 * tb)})
+        eng.close()
+
+    # ---- the reference's own method (Green-function ratio, quirks) for the CPU's tau ----
+    tau_ref = None
+    if not args.no_iat and not args.no_cpu:
+        cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
+                            sampling=V.IMPORTANCE, step_length=args.dt, precision=32,
+                            refresh_interval=args.refresh, num_walkers=args.walkers)
+        with V.Engine(cfg) as eng:
+            eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
+            eng.set_positions(pos)
+            eng.set_random_state(args.seed + 13, 0)
+            eng.run(5000, 0)
+            tau_ref = measure_iat(eng, args.iat_steps, args.iat_walkers)[0]
+
     cpu = None if args.no_cpu else cpu_baseline(args)
+    if cpu and tau_ref:
+        cpu["tau_int_blocking"] = tau_ref
+        cpu["iat_corrected_samples_per_s"] = cpu["value"] / (2 * tau_ref)
+        cpu["tau_note"] = ("tau of E_L for the reference's own method (its Green-function ratio "
+                           "and Pade ratio semantics) measured with the engine's compatible mode "
+                           "on the same equilibrated walkers")
 
     out = {
         "metric": "walker-steps/s and IAT-corrected samples/s at N=56 w=0.1, vs plain FP64",
@@ -286,18 +388,20 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 walk, f64 inverse rebuild and accumulation",
-        "data": "synthetic (walkers drawn from N(0, 1/omega^2) per coordinate, burned in)",
+        "data": "synthetic (walkers drawn from N(0, 1/omega^2) per coordinate, equilibrated)",
         "config": {"workload": f"2-D quantum dot N={args.N} omega={args.omega}, Coulomb, "
-                               "Slater(Hermite) x Gaussian x Pade-Jastrow, exact MH Langevin "
-                               f"dt={args.dt}, drift limit a={args.drift_limit}",
+                               "Slater(Hermite) x Gaussian x Pade-Jastrow (alpha=gamma=1), "
+                               f"exact MH Langevin dt={args.dt}, drift limit a={args.drift_limit}",
                    "walkers": args.walkers, "steps_per_iter": args.steps_per_iter,
-                   "refresh_interval": args.refresh, "burnin_steps": args.burnin},
+                   "refresh_interval": args.refresh, "equilibration_steps": args.equil_steps,
+                   "equilibration_dt": args.equil_dt},
         "kernel_walker_steps_per_s": kern_rate,
         "energy": E32,
         "acceptance": acc32,
         "iat": iat,
         "iat_corrected_samples_per_s": (value / (2 * iat["tau_int_blocking"])) if iat else None,
         "fp64": fp64,
+        "learned": learned,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "note": "VALU FP32 compute roofline (no MFMA in the walk kernel; walker "
@@ -308,6 +412,13 @@ def main():
     }
     if cpu:
         out["gpu_vs_cpu"] = value / cpu["value"]
+    if learned and fp64 and "iat_corrected_samples_per_s" in learned and \
+            "iat_corrected_samples_per_s" in fp64:
+        out["learned_vs_plain_fp64_iat_corrected"] = (learned["iat_corrected_samples_per_s"] /
+                                                      fp64["iat_corrected_samples_per_s"])
+        if cpu and "iat_corrected_samples_per_s" in cpu:
+            out["learned_vs_cpu_iat_corrected"] = (learned["iat_corrected_samples_per_s"] /
+                                                   cpu["iat_corrected_samples_per_s"])
     print(json.dumps(out))
 
 
