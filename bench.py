@@ -44,6 +44,11 @@ def bytes_per_step(N, n, b, D=2, a=1.0):
     return b * ((1 + a) * n * n + 2 * D * n * n + (2 * D + 1) * n + 3 * N * D)
 
 
+# HBM bytes the FP32 N=56 walk kernel moves per walker per launch (positions in
+# and out, accumulators), from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by
+# a 1 GiB copy (profiles/walk_kernel_f32_N56_w0.1_hbm_traffic.md)
+PMC_BYTES_PER_WALKER_LAUNCH = 2116.0
+
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -241,6 +246,23 @@ def lp_initial_weights(seed):
 V_LP_NPARAM = 5636
 
 
+def copy_workload(args):
+    """A device-to-device copy of --copy-mib MiB, 5 times: the reference reading for
+    FETCH_SIZE / WRITE_SIZE (each launch reads and writes exactly that many bytes)."""
+    import torch
+    n = args.copy_mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"workload": "copy", "mib": args.copy_mib, "bytes_read_per_launch": 4 * n,
+                      "bytes_written_per_launch": 4 * n, "GB_per_s": 8.0 * n / dt / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,8 +294,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-iat", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--workload", choices=["dot56", "rbmpj"], default="dot56",
-                    help="dot56: the headline N=56 workload; rbmpj: RBM x Slater x Pade N=12 H=24")
+    ap.add_argument("--workload", choices=["dot56", "rbmpj", "copy"], default="dot56",
+                    help="dot56: the headline N=56 workload; rbmpj: RBM x Slater x Pade N=12 H=24; "
+                         "copy: a plain device copy of --copy-mib MiB (calibrates HBM byte counters)")
+    ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -281,6 +305,9 @@ def main():
               "and no multi-GPU figure is produced (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "copy":
+        copy_workload(args)
+        return
     import vmachine_amd as V
     if V.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -403,11 +430,14 @@ def main():
         "fp64": fp64,
         "learned": learned,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": PMC_BYTES_PER_WALKER_LAUNCH * args.walkers,
                      "note": "VALU FP32 compute roofline (no MFMA in the walk kernel; walker "
                              "state is register-resident); algorithmic flops "
                              f"{F:.0f}/walker-step x {steps_per_launch} walker-steps per launch; "
-                             f"streaming-equivalent bytes {B4:.0f}/walker-step"},
+                             f"streaming-equivalent bytes {B4:.0f}/walker-step; traffic = HBM "
+                             "bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                             "(scaled by a 1 GiB copy) at N=56, 100 steps per launch"},
         "cpu_baseline": cpu,
     }
     if cpu:
