@@ -202,6 +202,120 @@ def rbmpj_workload(args, V):
     print(json.dumps(out))
 
 
+BATCH_DOTS = [(2, 1.0), (6, 1.0), (12, 1.0), (20, 0.5), (30, 0.5), (42, 0.28), (56, 0.1)]
+
+
+def batch_workload(args, V):
+    """value from the first of --batch-walkers (comma-separated walker counts
+    per system); every count is measured the same way."""
+    counts = [int(c) for c in str(args.batch_walkers).split(",")]
+    lines = [batch_measure(args, V, W) for W in counts]
+    out = lines[0]
+    out["other_walker_counts"] = lines[1:]
+    print(json.dumps(out))
+
+
+def batch_measure(args, V, W):
+    """Seven closed-shell dots N = 2..56 (Slater x Gaussian x Pade-Jastrow,
+    Coulomb, alpha = gamma = 1, exact Langevin dt --dt with drift limit, FP32
+    walk with an FP64 rebuild every --refresh steps), --batch-walkers walkers
+    each.  One optimisation iteration = --batch-steps steps per walker of every
+    system, per-system sums and the device ADAM step (learning rate
+    --batch-lr).  value = walker-steps/s of all systems together, one
+    persistent launch per refresh segment, the iteration replayed as a hipGraph,
+    shapes from the autotuner; beside it the same iterations with the default
+    shapes, without the graph, and as seven single-system engines run one after
+    another (host optimiser)."""
+    S, lr = args.batch_steps, args.batch_lr
+    systems = [dict(num_particles=N, omega=w, num_walkers=W, step_length=args.dt, alpha=1.0,
+                    gamma=1.0) for N, w in BATCH_DOTS]
+    kinds = [V.SLATER, V.GAUSSIAN, V.PADE]
+
+    def make_batch():
+        b = V.Batch(systems, kinds=kinds, sampling=V.IMPORTANCE_EXACT, quirks=0, precision=32,
+                    refresh_interval=args.refresh, drift_limit=args.drift_limit)
+        b.init_walkers(args.seed)
+        b.run(max(args.warmup, 1), S, 0, learning_rate=lr)
+        return b
+
+    def time_batch(b, use_graph):
+        t0 = time.perf_counter()
+        out, ms = b.run(args.steps, S, 0, learning_rate=lr, use_graph=use_graph)
+        wall = time.perf_counter() - t0
+        return wall, ms / 1000.0, out
+
+    total_ws = W * len(systems) * S * args.steps
+    res = {}
+    b = make_batch()
+    wall, kern, _ = time_batch(b, True)
+    res["default_shapes_graph"] = {"walker_steps_per_s": total_ws / wall,
+                                   "kernel_walker_steps_per_s": total_ws / kern}
+    print(f"batch default shapes: {total_ws / wall:.4g} walker-steps/s", file=sys.stderr, flush=True)
+    lanes, waves, ns = b.autotune(args.batch_tune_steps)
+    print(f"autotune lanes={lanes.tolist()} waves={waves.tolist()}", file=sys.stderr, flush=True)
+    wall, kern, out = time_batch(b, True)
+    res["graph"] = {"walker_steps_per_s": total_ws / wall, "kernel_walker_steps_per_s": total_ws / kern,
+                    "ms_per_step": 1000.0 * wall / args.steps}
+    wall_d, kern_d, _ = time_batch(b, False)
+    res["direct"] = {"walker_steps_per_s": total_ws / wall_d,
+                     "kernel_walker_steps_per_s": total_ws / kern_d}
+    energies = (out[-1, :, 0] / (W * S)).tolist()
+    params = b.parameters().tolist()
+    b.close()
+    print(f"batch autotuned graph: {res['graph']['walker_steps_per_s']:.4g}", file=sys.stderr,
+          flush=True)
+
+    # the same work as single-system engines, one after another, host optimiser
+    from vmachine_amd import host as H
+    engines = []
+    for s, (N, w) in enumerate(BATCH_DOTS):
+        cfg = V.make_config(N, omega=w, kinds=kinds, sampling=V.IMPORTANCE_EXACT,
+                            step_length=args.dt, quirks=0, drift_limit=args.drift_limit,
+                            precision=32, refresh_interval=args.refresh, num_walkers=W)
+        e = V.Engine(cfg)
+        P = np.array([[0.0], [1.0], [1.0]])
+        e.set_parameters(P)
+        e.init_walkers(args.seed + s)
+        engines.append([e, P, H.Optimizer("adam", 3, lr)])
+
+    def seq_iteration(it):
+        for rec in engines:
+            e, P, opt = rec
+            sums, g, ge, _ = e.run(S, 0, it, 1)
+            upd = opt.update(sums["energy"] / sums["samples"], g / sums["gradient_samples"],
+                             ge / sums["gradient_samples"])
+            rec[1] = P - upd.reshape(P.shape)
+            e.set_parameters(rec[1])
+
+    for it in range(max(args.warmup, 1)):
+        seq_iteration(it)
+    t0 = time.perf_counter()
+    for it in range(args.steps):
+        seq_iteration(max(args.warmup, 1) + it)
+    wall_s = time.perf_counter() - t0
+    for rec in engines:
+        rec[0].close()
+    res["sequential_engines"] = {"walker_steps_per_s": total_ws / wall_s}
+    value = res["graph"]["walker_steps_per_s"]
+    out = {"metric": "walker-steps/s, 7 quantum dots N=2..56 in one persistent launch",
+           "value": value, "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["graph"]["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 walk, f64 inverse rebuild and accumulation",
+           "data": "synthetic (walkers drawn from N(0, 1/omega^2) per system)",
+           "config": {"workload": "closed-shell 2-D dots N=2,6,12,20,30,42,56 (omega "
+                                  "1,1,1,0.5,0.5,0.28,0.1), Coulomb, Slater x Gaussian x "
+                                  "Pade-Jastrow, exact Langevin",
+                      "walkers_per_system": W, "steps_per_iteration": S, "dt": args.dt,
+                      "refresh_interval": args.refresh, "learning_rate": lr},
+           "autotune": {"lanes_per_walker": lanes.tolist(), "waves_per_tile": waves.tolist(),
+                        "ns_per_walker_step": ns.tolist()},
+           "forms": res,
+           "speedup_vs_sequential_engines": value / res["sequential_engines"]["walker_steps_per_s"],
+           "energies_last_iteration": energies, "alpha_gamma": params}
+    return out
+
+
 def equilibrate(V, args):
     """Walker positions at equilibrium: from N(0, 1/omega^2) per coordinate (ref
     randomnormal.cpp:17), exact Langevin with a large step (dt=--equil-dt, drift
@@ -294,9 +408,15 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-iat", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--workload", choices=["dot56", "rbmpj", "copy"], default="dot56",
+    ap.add_argument("--workload", choices=["dot56", "rbmpj", "batch", "copy"], default="dot56",
                     help="dot56: the headline N=56 workload; rbmpj: RBM x Slater x Pade N=12 H=24; "
+                         "batch: seven dots N=2..56 in one persistent launch; "
                          "copy: a plain device copy of --copy-mib MiB (calibrates HBM byte counters)")
+    ap.add_argument("--batch-walkers", default="8192,1024",
+                    help="walkers per system, comma-separated: one measurement each")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-lr", type=float, default=0.01)
+    ap.add_argument("--batch-tune-steps", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
 
@@ -314,6 +434,9 @@ def main():
         sys.exit(1)
     if args.workload == "rbmpj":
         rbmpj_workload(args, V)
+        return
+    if args.workload == "batch":
+        batch_workload(args, V)
         return
     n = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28}[args.N]
     pos = equilibrate(V, args)

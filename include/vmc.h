@@ -212,6 +212,64 @@ int vmc_lp_gradient(vmc_engine* e, const double* positions, int walkers, const i
 int vmc_lp_train(vmc_engine* e, int iterations, int walk_steps, double learning_rate,
                  double* history);
 
+/* ---- Batches: several quantum dots in one persistent launch ----
+ * Systems of different particle numbers and frequencies (Slater / Gaussian /
+ * Pade-Jastrow elements, the same for all) are walked by one launch whose
+ * workgroups take tiles of walkers from a work queue, largest systems first.
+ * vmc_batch_run runs whole optimisation iterations on the device: the walk,
+ * per-system sums, the reference's ADAM step on (alpha, gamma) of every system
+ * (ref src/Optimization/adam.cpp:12-38) and the advance of the random counter
+ * and gradient window; with use_graph the iteration is captured once as a
+ * hipGraph and replayed.  No counterpart in the reference (it runs one system
+ * per process). */
+#define VMC_BATCH_MAX_SYSTEMS 8
+typedef struct vmc_batch vmc_batch;
+typedef struct vmc_batch_system {
+    int num_particles;
+    double omega;
+    int num_walkers;
+    double step_length;   /* stepLength of this system */
+    double alpha, gamma;  /* initial Gaussian and Pade parameters */
+} vmc_batch_system;
+typedef struct vmc_batch_config {
+    int num_systems;
+    const vmc_batch_system* systems;
+    int num_elements;     /* element kinds shared by the systems (VMC_SLATER, VMC_GAUSSIAN, VMC_PADE) */
+    int kinds[6];
+    int interaction;
+    int sampling;         /* VMC_BRUTE_FORCE, VMC_IMPORTANCE or VMC_IMPORTANCE_EXACT */
+    unsigned quirks;
+    int precision;        /* 32 or 64 */
+    int refresh_interval;
+    double drift_limit;
+} vmc_batch_config;
+#define VMC_BATCH_OUT 16  /* per system and iteration: the 14 VMC_EVAL-order sums, then alpha, gamma */
+
+int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out);
+void vmc_batch_destroy(vmc_batch* b);
+/* System s draws from seed + s (its walkers retrace a single-system engine
+ * created with that seed). */
+int vmc_batch_init_walkers(vmc_batch* b, uint64_t seed);
+int vmc_batch_set_positions(vmc_batch* b, int system, const double* positions, int walkers, int dof);
+int vmc_batch_get_positions(vmc_batch* b, int system, double* positions, int walkers, int dof);
+/* alpha_gamma [num_systems][2]: the Gaussian alpha and Pade gamma of every
+ * system after the last vmc_batch_run (the device optimiser's values). */
+int vmc_batch_get_parameters(vmc_batch* b, double* alpha_gamma);
+/* Kernel shape of one system: lanes per walker (a power of two >= the
+ * orbitals per spin) and waves per tile. */
+int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker, int waves_per_tile);
+/* Times each system's candidate shapes on its own walkers (positions and
+ * random state are restored) and keeps the fastest; lanes_out / waves_out
+ * [num_systems] receive the choices, ns_per_step_out [num_systems] (may be
+ * null) their time per walker-step. */
+int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, int* waves_out,
+                       double* ns_per_step_out);
+/* `iterations` optimisation iterations of equilibration + steps steps each.
+ * out [iterations][num_systems][VMC_BATCH_OUT]; kernel_ms (may be null) = the
+ * device time of all iterations.  learning_rate 0 keeps the parameters. */
+int vmc_batch_run(vmc_batch* b, int iterations, int steps, int equilibration, int batches,
+                  double learning_rate, int use_graph, double* out, double* kernel_ms);
+
 /* Parity entry point for the random-number protocol: the draws of steps
  * counter0 .. counter0+n-1 of one walker exactly as the walk computes them in
  * the given precision (32 or 64): coordinate index in [0, dof), standard

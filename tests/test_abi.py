@@ -48,3 +48,34 @@ def test_no_device_is_an_error_not_a_fallback():
         pytest.skip("a GPU is visible")
     with pytest.raises(V.VmcError, match="no HIP device"):
         V.Engine(V.make_config(6))
+
+
+def _dots(*Ns):
+    return [dict(num_particles=N, omega=1.0, num_walkers=64, step_length=0.01, alpha=1.0,
+                 gamma=1.0) for N in Ns]
+
+
+@pytest.mark.parametrize("systems,kw,msg", [
+    (_dots(), {}, "systems"),
+    (_dots(*([2] * 9)), {}, "systems"),
+    (_dots(2, 8), {}, "system 1: slaterDeterminant needs a closed shell"),
+    (_dots(2, 6), dict(kinds=(V.SLATER, V.RBM_GAUSSIAN)), "Slater, Gaussian and Pade"),
+    (_dots(2, 6), dict(kinds=(V.GAUSSIAN, V.GAUSSIAN)), "twice"),
+    (_dots(2, 6), dict(sampling=V.LEARNED), "brute force or Langevin"),
+    (_dots(2, 6), dict(precision=16), "precision"),
+    (_dots(2, 6), dict(drift_limit=-1.0), "drift_limit"),
+    ([dict(num_particles=57, omega=1.0, num_walkers=8, step_length=0.01, alpha=1.0, gamma=1.0)],
+     dict(kinds=(V.GAUSSIAN, V.PADE)), "numParticles"),
+    ([dict(num_particles=6, omega=0.0, num_walkers=8, step_length=0.01, alpha=1.0, gamma=1.0)],
+     {}, "system 0: num_walkers"),
+])
+def test_invalid_batches_are_rejected(systems, kw, msg):
+    with pytest.raises(V.VmcError, match=msg):
+        V.Batch(systems, **kw)
+
+
+def test_batch_without_device_is_an_error():
+    if V.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(V.VmcError, match="no HIP device"):
+        V.Batch(_dots(2, 6, 12))

@@ -1,0 +1,253 @@
+"""The batch form: quantum dots of different electron counts walked by one
+persistent launch, an optimisation iteration per hipGraph replay, the
+optimiser on the device (vmc_batch_* in include/vmc.h).
+
+Pinned three ways:
+ * every system of a batch retraces the FP64 oracle's chains (Philox stream
+   seed + s for system s) step for step, as a single-system engine does;
+ * per system, the batch's sums and final positions equal those of a
+   single-system engine seeded seed + s, in FP64 and FP32, also after the
+   kernel shape (lanes per walker, waves per tile) is changed;
+ * graph replay and direct launches give bit-identical results, and the
+   device ADAM step reproduces the host optimiser (ref
+   src/Optimization/adam.cpp:12-38) iteration by iteration.
+Tolerances: FP64 1e-9 relative (summation order differs between the batch's
+per-system reduction and the engine's), FP32 as stated in each test.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import OMEGA_FOR_N, engine_config, oracle_config
+
+pytestmark = pytest.mark.gpu
+
+KINDS_SGP = [O.SLATER, O.GAUSSIAN, O.PADE]
+# (N, walkers, dt): deliberately not in cost order, ragged walker counts
+SYSTEMS = [(12, 37, 0.02), (2, 100, 0.1), (6, 64, 0.05), (20, 45, 0.01)]
+LANES = {1: 1, 3: 4, 6: 8, 10: 16, 15: 16, 21: 32, 28: 32}   # orbitals per spin -> default K
+ORBITALS = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28}
+
+
+def _require_gpu():
+    import vmachine_amd as V
+    if V.device_count() < 1:
+        pytest.fail("no GPU visible: GPU tests must run on the MI355X box")
+    return V
+
+
+def _systems(rng, systems=SYSTEMS):
+    out = []
+    for N, W, dt in systems:
+        out.append(dict(num_particles=N, omega=OMEGA_FOR_N[N], num_walkers=W, step_length=dt,
+                        alpha=1.0 + 0.1 * rng.standard_normal(), gamma=0.3 + 0.7 * rng.random()))
+    return out
+
+
+def _batch(V, systems, sampling, precision=64, quirks=0b1100, drift=0.0, refresh=0):
+    return V.Batch(systems, kinds=KINDS_SGP, sampling=sampling, precision=precision, quirks=quirks,
+                   drift_limit=drift, refresh_interval=refresh)
+
+
+def _params(y):
+    return np.array([[0.0], [y["alpha"]], [y["gamma"]]])
+
+
+def _engine(V, y, sampling, precision=64, drift=0.0, refresh=0):
+    cfg = engine_config(y["num_particles"], y["omega"], KINDS_SGP, sampling, y["step_length"],
+                        precision=precision, walkers=y["num_walkers"], refresh=refresh,
+                        drift_limit=drift)
+    e = V.Engine(cfg)
+    e.set_parameters(_params(y))
+    return e
+
+
+SUM_KEYS = ["energy", "energy2", "kinetic", "external", "interaction", "kinetic2", "external2",
+            "interaction2", "accepted"]
+
+
+def _check_sums(out_s, sums, g, ge, rtol):
+    for k, key in enumerate(SUM_KEYS):
+        assert abs(out_s[k] - sums[key]) <= rtol * max(1.0, abs(sums[key])), (key, out_s[k], sums[key])
+    for k, ref in [(9, g[1, 0]), (10, g[2, 0]), (11, ge[1, 0]), (12, ge[2, 0])]:
+        assert abs(out_s[k] - ref) <= rtol * max(1.0, abs(ref)), (k, out_s[k], ref)
+    assert out_s[13] == sums["nonfinite"]
+
+
+@pytest.mark.parametrize("sampling,drift", [(O.IMPORTANCE_COMPAT, 0.0),
+                                            (O.IMPORTANCE_EXACT, 1.0),
+                                            (O.BRUTE_FORCE, 0.0)])
+def test_batch_retraces_the_oracle(sampling, drift):
+    V = _require_gpu()
+    rng = np.random.default_rng(41 + sampling)
+    systems = _systems(rng)
+    if sampling == O.BRUTE_FORCE:
+        for y in systems:
+            y["step_length"] = 0.5 / np.sqrt(y["omega"])
+    steps, equil, seed = 120, 10, 500 + sampling
+    print(f"batch vs oracle sampling={sampling}", flush=True)
+    pos0 = []
+    with _batch(V, systems, sampling, drift=drift) as b:
+        b.init_walkers(seed)   # the random key: system s draws from seed + s
+        for s, y in enumerate(systems):
+            N, W = y["num_particles"], y["num_walkers"]
+            p = np.stack([O.initial_positions(seed + s, w, 2 * N, y["omega"]) for w in range(W)])
+            p = p * np.sqrt(y["omega"])   # a gently spread start, as in the single-engine test
+            pos0.append(p)
+            b.set_positions(s, p)
+        out, _ = b.run(1, steps, equil)
+        gpos = [b.get_positions(s) for s in range(len(systems))]
+    for s, y in enumerate(systems):
+        N, W = y["num_particles"], y["num_walkers"]
+        ocfg = oracle_config(N, y["omega"], KINDS_SGP, sampling, y["step_length"],
+                             drift_limit=drift)
+        opos = pos0[s].copy()
+        counters = np.zeros(W, dtype=np.uint64)
+        acc, og, oge = O.run_many(ocfg, _params(y), opos, seed + s, counters, steps, equil, 0, 1,
+                                  threads=8)
+        o = out[0, s]
+        assert o[8] == acc[8], (s, o[8], acc[8])
+        for k in range(8):
+            assert abs(o[k] - acc[k]) <= 1e-9 * max(1.0, abs(acc[k])), (s, k, o[k], acc[k])
+        for k, ref in [(9, og[1, 0]), (10, og[2, 0]), (11, oge[1, 0]), (12, oge[2, 0])]:
+            assert abs(o[k] - ref) <= 1e-9 * max(1.0, abs(ref)), (s, k, o[k], ref)
+        assert np.max(np.abs(gpos[s] - opos)) < 1e-9 * max(1.0, np.max(np.abs(opos))), s
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_batch_matches_single_engines(precision):
+    """Two iterations (gradient window of iteration 1 of 2 batches in the
+    second) with a refresh every 50 steps; FP32 sums within 1e-5 relative."""
+    V = _require_gpu()
+    rng = np.random.default_rng(77)
+    systems = _systems(rng)
+    seed, steps, equil = 1234, 200, 20
+    sampling = O.IMPORTANCE_EXACT
+    print(f"batch vs engines precision={precision}", flush=True)
+    with _batch(V, systems, sampling, precision=precision, refresh=50) as b:
+        b.init_walkers(seed)
+        out0, _ = b.run(1, steps, equil, batches=2, use_graph=False)
+        out1, _ = b.run(1, steps, equil, batches=2, use_graph=True)
+        bpos = [b.get_positions(s) for s in range(len(systems))]
+    rtol = 1e-9 if precision == 64 else 1e-5
+    for s, y in enumerate(systems):
+        with _engine(V, y, sampling, precision=precision, refresh=50) as e:
+            e.init_walkers(seed + s)
+            r0 = e.run(steps, equil, 0, 2)
+            r1 = e.run(steps, equil, 1, 2)
+            epos = e.positions()
+        _check_sums(out0[0, s], *r0[:3], rtol)
+        _check_sums(out1[0, s], *r1[:3], rtol)
+        if precision == 64:
+            assert np.max(np.abs(bpos[s] - epos)) < 1e-9 * max(1.0, np.max(np.abs(epos)))
+        else:
+            # same code and draws: FP32 chains agree to rounding unless an
+            # acceptance test sits on its threshold
+            close = np.all(np.abs(bpos[s] - epos) < 1e-4, axis=1)
+            assert close.mean() >= 0.95, (s, close.mean())
+
+
+def test_graph_replay_equals_direct_launches():
+    V = _require_gpu()
+    rng = np.random.default_rng(5)
+    systems = _systems(rng)
+    res = []
+    for use_graph in (True, False):
+        with _batch(V, systems, O.IMPORTANCE_EXACT, precision=32, refresh=40) as b:
+            b.init_walkers(99)
+            out, ms = b.run(4, 100, 10, batches=2, learning_rate=0.05, use_graph=use_graph)
+            assert ms > 0
+            res.append((out, [b.get_positions(s) for s in range(len(systems))], b.parameters()))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    for a, c in zip(res[0][1], res[1][1]):
+        np.testing.assert_array_equal(a, c)
+    np.testing.assert_array_equal(res[0][2], res[1][2])
+
+
+def test_device_adam_matches_host_optimiser():
+    """Three iterations of lr 0.05: (alpha, gamma) after every device ADAM step
+    against the host optimiser driven by single-engine sums."""
+    V = _require_gpu()
+    from vmachine_amd import host as H
+    rng = np.random.default_rng(9)
+    systems = _systems(rng)
+    seed, steps, iters, lr, B = 4321, 200, 3, 0.05, 2
+    with _batch(V, systems, O.IMPORTANCE_EXACT) as b:
+        b.init_walkers(seed)
+        out, _ = b.run(iters, steps, 0, batches=B, learning_rate=lr)
+        final = b.parameters()
+    for s, y in enumerate(systems):
+        P = _params(y)
+        opt = H.Optimizer("adam", P.size, lr)
+        with _engine(V, y, O.IMPORTANCE_EXACT) as e:
+            e.init_walkers(seed + s)
+            for it in range(iters):
+                sums, g, ge, _ = e.run(steps, 0, it, B)
+                assert abs(out[it, s, 0] - sums["energy"]) <= 1e-9 * abs(sums["energy"])
+                avgE = sums["energy"] / sums["samples"]
+                upd = opt.update(avgE, g / sums["gradient_samples"], ge / sums["gradient_samples"])
+                P = P - upd.reshape(P.shape)
+                e.set_parameters(P)
+                np.testing.assert_allclose(out[it, s, 14:16], [P[1, 0], P[2, 0]], rtol=1e-9,
+                                           atol=1e-12)
+        np.testing.assert_allclose(final[s], [P[1, 0], P[2, 0]], rtol=1e-9, atol=1e-12)
+
+
+def test_shapes_and_autotune():
+    """A changed shape (2K lanes, 2 waves per tile) walks the same chains; the
+    autotuner picks valid shapes and leaves positions untouched."""
+    V = _require_gpu()
+    rng = np.random.default_rng(3)
+    systems = _systems(rng)
+    seed, steps = 77, 150
+    with _batch(V, systems, O.IMPORTANCE_EXACT) as b:
+        b.init_walkers(seed)
+        ref, _ = b.run(1, steps)
+        ref_pos = [b.get_positions(s) for s in range(len(systems))]
+    with _batch(V, systems, O.IMPORTANCE_EXACT) as b:
+        for s, y in enumerate(systems):
+            K = LANES[ORBITALS[y["num_particles"]]]
+            b.set_shape(s, 2 * K, 2)
+        b.init_walkers(seed)
+        out, _ = b.run(1, steps)
+        for s in range(len(systems)):
+            np.testing.assert_allclose(out[0, s, :13], ref[0, s, :13], rtol=1e-9, atol=1e-9)
+            assert np.max(np.abs(b.get_positions(s) - ref_pos[s])) < 1e-9
+    with _batch(V, systems, O.IMPORTANCE_EXACT, precision=32) as b:
+        b.init_walkers(seed)
+        before = [b.get_positions(s) for s in range(len(systems))]
+        lanes, waves, ns = b.autotune(30)
+        print("autotune", lanes, waves, ns, flush=True)
+        for s, y in enumerate(systems):
+            K = LANES[ORBITALS[y["num_particles"]]]
+            assert lanes[s] in (K, 2 * K) and waves[s] in (1, 2, 4) and ns[s] > 0
+            np.testing.assert_array_equal(b.get_positions(s), before[s])
+        out, _ = b.run(2, 100, 10)
+        assert np.all(np.isfinite(out))
+        for s, y in enumerate(systems):
+            assert out[1, s, 8] > 0.5 * y["num_walkers"] * 100   # acceptance
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_uneven_dots_zero_variance(precision):
+    """The seven closed shells N = 2..56 in one launch, non-interacting,
+    Slater x Gaussian with alpha = 1: every local energy is the exact
+    E0 = 2 sum (s+1)^2 omega, so every system's mean and variance are fixed."""
+    V = _require_gpu()
+    from tests.helpers import zero_variance_energy
+    Ns = [2, 6, 12, 20, 30, 42, 56]
+    omegas = [1.0, 0.5, 1.0, 0.28, 0.5, 1.0, 0.1]
+    systems = [dict(num_particles=N, omega=w, num_walkers=96 + 7 * i, step_length=0.01,
+                    alpha=1.0, gamma=0.0) for i, (N, w) in enumerate(zip(Ns, omegas))]
+    with V.Batch(systems, kinds=[O.SLATER, O.GAUSSIAN], sampling=O.IMPORTANCE_EXACT,
+                 precision=precision, interaction=False, quirks=0b1100, refresh_interval=100) as b:
+        b.init_walkers(11)
+        out, _ = b.run(2, 200, 50)
+    tol = 1e-10 if precision == 64 else 1e-5
+    for s, y in enumerate(systems):
+        E0 = zero_variance_energy(y["num_particles"], y["omega"])
+        M = y["num_walkers"] * 200
+        E, E2 = out[1, s, 0] / M, out[1, s, 1] / M
+        assert abs(E - E0) <= tol * E0, (y["num_particles"], E, E0)
+        assert abs(E2 - E0 * E0) <= 4 * tol * E0 * E0
+        assert out[1, s, 8] > 0.5 * M

@@ -89,6 +89,17 @@ struct WalkParams {
     int init_qf;        // 1: set the stored drift to the drift at the loaded positions
 };
 
+// the values of one walk launch that change between launches and iterations
+// while the rest of WalkParams stays fixed (the batch kernel reads them from
+// device state; the single-system kernel copies them from its WalkParams)
+struct WalkDyn {
+    double alpha, gamma;  // Gaussian alpha, Pade gamma
+    uint64_t counter0;
+    int t_begin, t_end, acc_add;
+    int window_lo, window_hi;
+    int init_qf;
+};
+
 // training of the learned proposal: one sample per walker (vmc_learn.hip)
 struct LpTrainParams {
     int B, N;               // samples, electrons
@@ -117,6 +128,40 @@ struct EvalOut {
     double* g1;        // after
     int rows[6];       // element kind of each parameter row
     int num_elements, maxpar;
+};
+
+// ---- batch of systems walked by one persistent launch (vmc_batch.hip) ----
+enum : int { kBatchAlpha = kNumWalkerAcc, kBatchGamma, kBatchOut };
+
+struct BatchSystem {
+    WalkParams P;       // the system's parameters and buffers; per-iteration fields set in-kernel
+    int shape;          // instantiation id (np, K)
+    int np, K;
+    int waves;          // blocks (waves) per tile
+    int tile0, ntiles;  // the system's tiles in the work queue
+    int walker0;        // first row of the system in the shared walker buffers
+    int user;           // the system's index in the caller's list
+};
+
+// the static description of every system of a batch (device memory, written
+// by the host before a run, read-only for the kernels)
+constexpr int kBatchMaxSystems = 8;
+struct BatchTable {
+    BatchSystem sys[kBatchMaxSystems];   // in tile order (costliest first)
+    int nsys, total_tiles;
+};
+
+struct BatchState {
+    uint64_t counter0;  // random counter of the iteration's first step
+    int iteration;      // optimisation iteration (gradient window = iteration % batches)
+    int batches;        // gradient batches
+    int fresh;          // 1: set the stored drift from the positions (cleared after an iteration)
+    double learning_rate;
+};
+
+struct BatchOpt {
+    double m[2], v[2];  // ADAM moments of alpha and gamma
+    int step;
 };
 
 // which kernel instantiation runs a configuration: np particle slots per spin

@@ -33,6 +33,15 @@ hipError_t launch_lp_adam(float* theta, uint16_t* bf16, float* m, float* v, cons
 int lp_wgrad_rows(int B);
 int lp_act_floats();
 int lp_delta_floats();
+hipError_t launch_batch_walk(int precision, const BatchTable* T, int* queue, int grid, size_t lds,
+                             int t_begin, int t_end, int acc_add, const BatchState* st,
+                             const double* theta, hipStream_t stream);
+hipError_t launch_batch_tail(const BatchTable* T, int nsys, BatchOpt* opt, double* theta,
+                             const double* acc, double* out, BatchState* st, int it0,
+                             int steps_total, hipStream_t stream);
+int batch_shape_id(int np, int K);
+size_t batch_lds_bytes(int np, int K);
+int batch_resident_per_cu(int precision, size_t lds);
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
                                  hipStream_t stream);
@@ -903,6 +912,464 @@ int vmc_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, 
     (void)hipFree(d_k);
     (void)hipFree(d_f);
     return VMC_OK;
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+struct vmc_batch {
+    vmc_batch_config cfg{};
+    std::vector<vmc_batch_system> sys;     // caller order
+    std::vector<vmc::BatchSystem> hs;      // tile order (costliest first)
+    std::vector<int> dev_of;               // caller index -> tile-order index
+    std::vector<double> theta;             // [tile order][alpha, gamma]
+    int total_walkers = 0, total_tiles = 0, grid = 0, cus = 0;
+    size_t lds = 0;
+    uint64_t seed = 0, counter = 0;
+    int iteration = 0;
+    bool walkers_set = false, fresh = true;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    vmc::BatchTable* d_table = nullptr;    // [2]: the run's table, the autotuner's
+    vmc::BatchState* d_st = nullptr;
+    vmc::BatchOpt* d_opt = nullptr;
+    double *d_theta = nullptr, *d_pos = nullptr, *d_qf = nullptr, *d_acc = nullptr;
+    int* d_queue = nullptr;
+    std::vector<size_t> pos_off;           // caller index -> first double of its positions
+
+    ~vmc_batch()
+    {
+        for (void* p : {(void*)d_table, (void*)d_st, (void*)d_opt, (void*)d_theta, (void*)d_pos, (void*)d_qf,
+                        (void*)d_acc, (void*)d_queue})
+            if (p) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int batch_tiles(const vmc::BatchSystem& B)
+{
+    const int G = 64 / B.K;
+    return ((B.P.W + G - 1) / G + B.waves - 1) / B.waves;
+}
+
+// tiles of every system for its current shape; LDS and the persistent grid
+void batch_plan(vmc_batch* b)
+{
+    int t = 0;
+    size_t lds = 0;
+    for (auto& B : b->hs) {
+        B.ntiles = batch_tiles(B);
+        B.tile0 = t;
+        t += B.ntiles;
+        lds = std::max(lds, vmc::batch_lds_bytes(B.np, B.K));
+    }
+    b->total_tiles = t;
+    b->lds = lds;
+    b->grid = b->cus * std::max(1, vmc::batch_resident_per_cu(b->cfg.precision, lds));
+}
+
+vmc::BatchTable batch_table(const vmc_batch* b)
+{
+    vmc::BatchTable T{};
+    T.nsys = (int)b->hs.size();
+    T.total_tiles = b->total_tiles;
+    for (int d = 0; d < T.nsys; d++) T.sys[d] = b->hs[d];
+    return T;
+}
+
+// one iteration's walk launches (one per refresh segment) over the table in
+// d_table[0]
+int batch_enqueue_walk(vmc_batch* b, int total)
+{
+    const int seg = b->cfg.refresh_interval > 0 ? b->cfg.refresh_interval : total;
+    for (int t0 = 0; t0 < total; t0 += seg) {
+        const int t1 = t0 + seg < total ? t0 + seg : total;
+        HIP_TRY(vmc::launch_batch_walk(b->cfg.precision, b->d_table, b->d_queue, b->grid, b->lds,
+                                       t0, t1, t0 > 0 ? 1 : 0, b->d_st, b->d_theta, b->stream));
+    }
+    return VMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
+{
+    if (!cfg || !out || !cfg->systems) return fail(VMC_E_INVALID, "null argument");
+    *out = nullptr;
+    const vmc_batch_config& c = *cfg;
+    if (c.num_systems < 1 || c.num_systems > vmc::kBatchMaxSystems)
+        return fail(VMC_E_INVALID, "1.." + std::to_string(vmc::kBatchMaxSystems) + " systems");
+    if (c.num_elements < 1 || c.num_elements > 3) return fail(VMC_E_INVALID, "1..3 elements");
+    bool has_slater = false;
+    for (int e = 0; e < c.num_elements; e++) {
+        const int k = c.kinds[e];
+        if (k != VMC_GAUSSIAN && k != VMC_SLATER && k != VMC_PADE)
+            return fail(VMC_E_INVALID, "batches take Slater, Gaussian and Pade-Jastrow elements");
+        for (int f = 0; f < e; f++)
+            if (c.kinds[f] == k) return fail(VMC_E_INVALID, "an element appears twice");
+        has_slater |= k == VMC_SLATER;
+    }
+    if (c.sampling < VMC_BRUTE_FORCE || c.sampling > VMC_IMPORTANCE_EXACT)
+        return fail(VMC_E_INVALID, "batches take brute force or Langevin sampling");
+    if (c.precision != 32 && c.precision != 64) return fail(VMC_E_INVALID, "precision 32 or 64");
+    if (c.refresh_interval < 0 || !(c.drift_limit >= 0))
+        return fail(VMC_E_INVALID, "refresh_interval >= 0 and drift_limit >= 0");
+    vmc_batch* b = new vmc_batch();
+    auto cleanup = [&](int code) {
+        delete b;
+        return code;
+    };
+    b->cfg = c;
+    b->sys.assign(c.systems, c.systems + c.num_systems);
+    b->cfg.systems = nullptr;
+    std::vector<double> cost(c.num_systems);
+    std::vector<vmc::BatchSystem> hs(c.num_systems);
+    size_t pos = 0, acc = 0;
+    b->pos_off.resize(c.num_systems);
+    for (int s = 0; s < c.num_systems; s++) {
+        const vmc_batch_system& y = b->sys[s];
+        const std::string tag = "system " + std::to_string(s) + ": ";
+        if (y.num_walkers < 1 || y.num_walkers > (1 << 24) || !(y.omega > 0) ||
+            !(y.step_length > 0))
+            return cleanup(fail(VMC_E_INVALID, tag + "num_walkers in 1..2^24, omega and step_length positive"));
+        int np = -1;
+        if (has_slater) {
+            np = closed_shell_orbitals(y.num_particles);
+            if (np < 0) return cleanup(fail(VMC_E_INVALID, tag + "slaterDeterminant needs a closed shell"));
+        } else if (y.num_particles >= 1 && y.num_particles <= 56) {
+            const int half = (y.num_particles + 1) / 2;
+            for (int v : kNP)
+                if (v >= half) { np = v; break; }
+        }
+        if (np < 0) return cleanup(fail(VMC_E_INVALID, tag + "numParticles must be in 1..56"));
+        int K = 0;
+        for (size_t i = 0; i < sizeof(kNP) / sizeof(kNP[0]); i++)
+            if (kNP[i] == np) K = kLanes[i];
+        vmc::BatchSystem& B = hs[s];
+        B = vmc::BatchSystem{};
+        vmc::WalkParams& P = B.P;
+        P.N = y.num_particles;
+        P.n_up = y.num_particles / 2;
+        P.omega = y.omega;
+        for (int e = 0; e < c.num_elements; e++) {
+            P.has_gauss |= c.kinds[e] == VMC_GAUSSIAN;
+            P.has_slater |= c.kinds[e] == VMC_SLATER;
+            P.has_pade |= c.kinds[e] == VMC_PADE;
+        }
+        P.coulomb = c.interaction ? 1 : 0;
+        P.alpha = y.alpha;   // the walk reads the device copy of (alpha, gamma)
+        P.gamma = y.gamma;
+        P.a_same = 1.0 / 3.0;   // 2-D spin-parallel Pade a (ref padejastrow.cpp:143-146)
+        P.a_anti = 1.0;
+        P.ND = 2 * y.num_particles;
+        P.sampling = c.sampling;
+        P.dt = y.step_length;
+        P.q2_upper = (c.quirks & VMC_Q2_PADE_RATIO_UPPER) ? 1 : 0;
+        P.q3_half = (c.quirks & VMC_Q3_PADE_GAMMA_HALF) ? 1 : 0;
+        P.drift_a = c.drift_limit;
+        P.W = y.num_walkers;
+        B.np = np;
+        B.K = K;
+        B.shape = vmc::batch_shape_id(np, K);
+        B.waves = 1;
+        B.user = s;
+        B.walker0 = (int)acc;
+        b->pos_off[s] = pos;
+        pos += (size_t)y.num_walkers * 2 * y.num_particles;
+        acc += (size_t)y.num_walkers;
+        cost[s] = double(y.num_particles) * y.num_particles * y.num_walkers;
+    }
+    b->total_walkers = (int)acc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return cleanup(fail(VMC_E_DEVICE, "no HIP device visible"));
+    hipError_t err;
+    int dev = 0;
+    if ((err = hipGetDevice(&dev)) != hipSuccess ||
+        (err = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
+        (err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (err = hipEventCreate(&b->ev0)) != hipSuccess || (err = hipEventCreate(&b->ev1)) != hipSuccess ||
+        (err = hipMalloc(&b->d_pos, sizeof(double) * pos)) != hipSuccess ||
+        (err = hipMalloc(&b->d_qf, sizeof(double) * pos)) != hipSuccess ||
+        (err = hipMalloc(&b->d_acc, sizeof(double) * acc * vmc::kNumWalkerAcc)) != hipSuccess ||
+        (err = hipMalloc(&b->d_theta, sizeof(double) * 2 * c.num_systems)) != hipSuccess ||
+        (err = hipMalloc(&b->d_table, 2 * sizeof(vmc::BatchTable))) != hipSuccess ||
+        (err = hipMalloc(&b->d_st, sizeof(vmc::BatchState))) != hipSuccess ||
+        (err = hipMalloc(&b->d_opt, sizeof(vmc::BatchOpt) * c.num_systems)) != hipSuccess ||
+        (err = hipMalloc(&b->d_queue, sizeof(int))) != hipSuccess)
+        return cleanup(fail(VMC_E_DEVICE, std::string("batch set-up: ") + hipGetErrorString(err)));
+    for (int s = 0; s < c.num_systems; s++) {
+        vmc::WalkParams& P = hs[s].P;
+        P.pos = b->d_pos + b->pos_off[s];
+        P.qf = (c.sampling == VMC_IMPORTANCE) ? b->d_qf + b->pos_off[s] : nullptr;
+        P.acc = b->d_acc + (size_t)hs[s].walker0 * vmc::kNumWalkerAcc;
+    }
+    // tile order: costliest systems first, so that the small ones fill the tail
+    std::vector<int> order(c.num_systems);
+    for (int s = 0; s < c.num_systems; s++) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    b->dev_of.resize(c.num_systems);
+    for (int i = 0; i < c.num_systems; i++) {
+        b->hs.push_back(hs[order[i]]);
+        b->dev_of[order[i]] = i;
+        b->theta.push_back(hs[order[i]].P.alpha);
+        b->theta.push_back(hs[order[i]].P.gamma);
+    }
+    batch_plan(b);
+    // ADAM moments start at one (ref src/Optimization/adam.cpp:17-19)
+    std::vector<vmc::BatchOpt> opt(c.num_systems, vmc::BatchOpt{{1.0, 1.0}, {1.0, 1.0}, 0});
+    if ((err = hipMemcpy(b->d_opt, opt.data(), sizeof(vmc::BatchOpt) * opt.size(),
+                         hipMemcpyHostToDevice)) != hipSuccess ||
+        (err = hipMemcpy(b->d_theta, b->theta.data(), sizeof(double) * b->theta.size(),
+                         hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(fail(VMC_E_DEVICE, std::string("batch set-up: ") + hipGetErrorString(err)));
+    *out = b;
+    g_err.clear();
+    return VMC_OK;
+}
+
+void vmc_batch_destroy(vmc_batch* b) { delete b; }
+
+int vmc_batch_init_walkers(vmc_batch* b, uint64_t seed)
+{
+    if (!b) return fail(VMC_E_INVALID, "null batch");
+    for (const auto& B : b->hs)
+        HIP_TRY(vmc::launch_init_positions(b->d_pos + b->pos_off[B.user], B.P.W, B.P.ND, B.P.omega,
+                                           seed + (uint64_t)B.user, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->seed = seed;
+    b->counter = 0;
+    b->iteration = 0;
+    b->walkers_set = true;
+    b->fresh = true;
+    return VMC_OK;
+}
+
+static int batch_check_system(vmc_batch* b, int system, const double* p, int walkers, int dof)
+{
+    if (!b || !p) return fail(VMC_E_INVALID, "null argument");
+    if (system < 0 || system >= (int)b->sys.size()) return fail(VMC_E_INVALID, "no such system");
+    const auto& y = b->sys[system];
+    if (walkers != y.num_walkers || dof != 2 * y.num_particles)
+        return fail(VMC_E_INVALID, "expected the system's num_walkers x (num_particles*2) values");
+    return VMC_OK;
+}
+
+int vmc_batch_set_positions(vmc_batch* b, int system, const double* positions, int walkers, int dof)
+{
+    if (int rc = batch_check_system(b, system, positions, walkers, dof)) return rc;
+    HIP_TRY(hipMemcpy(b->d_pos + b->pos_off[system], positions, sizeof(double) * walkers * dof,
+                      hipMemcpyHostToDevice));
+    b->walkers_set = true;
+    b->fresh = true;
+    return VMC_OK;
+}
+
+int vmc_batch_get_positions(vmc_batch* b, int system, double* positions, int walkers, int dof)
+{
+    if (int rc = batch_check_system(b, system, positions, walkers, dof)) return rc;
+    HIP_TRY(hipMemcpy(positions, b->d_pos + b->pos_off[system], sizeof(double) * walkers * dof,
+                      hipMemcpyDeviceToHost));
+    return VMC_OK;
+}
+
+int vmc_batch_get_parameters(vmc_batch* b, double* alpha_gamma)
+{
+    if (!b || !alpha_gamma) return fail(VMC_E_INVALID, "null argument");
+    for (size_t d = 0; d < b->hs.size(); d++) {
+        alpha_gamma[2 * b->hs[d].user] = b->theta[2 * d];
+        alpha_gamma[2 * b->hs[d].user + 1] = b->theta[2 * d + 1];
+    }
+    return VMC_OK;
+}
+
+int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker, int waves_per_tile)
+{
+    if (!b) return fail(VMC_E_INVALID, "null batch");
+    if (system < 0 || system >= (int)b->sys.size()) return fail(VMC_E_INVALID, "no such system");
+    vmc::BatchSystem& B = b->hs[b->dev_of[system]];
+    const int id = vmc::batch_shape_id(B.np, lanes_per_walker);
+    if (id < 0) return fail(VMC_E_INVALID, "no batch kernel shape for these lanes per walker");
+    if (waves_per_tile < 1 || waves_per_tile > 64) return fail(VMC_E_INVALID, "waves_per_tile in 1..64");
+    B.K = lanes_per_walker;
+    B.shape = id;
+    B.waves = waves_per_tile;
+    batch_plan(b);
+    return VMC_OK;
+}
+
+int vmc_batch_run(vmc_batch* b, int iterations, int steps, int equilibration, int batches,
+                  double learning_rate, int use_graph, double* out, double* kernel_ms)
+{
+    if (!b) return fail(VMC_E_INVALID, "null batch");
+    if (!b->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
+    if (iterations < 1 || steps < 1 || equilibration < 0 || batches < 1 || !(learning_rate >= 0))
+        return fail(VMC_E_INVALID,
+                    "iterations, steps, batches >= 1; equilibration, learning_rate >= 0");
+    const int nsys = (int)b->hs.size();
+    for (auto& B : b->hs) {
+        B.P.seed = b->seed + (uint64_t)B.user;   // system s retraces an engine seeded seed + s
+        B.P.steps = steps;
+        B.P.equil = equilibration;
+    }
+    const vmc::BatchTable T = batch_table(b);
+    const int it0 = b->iteration;
+    const int total = steps + equilibration;
+    const vmc::BatchState st{b->counter, it0, batches, b->fresh ? 1 : 0, learning_rate};
+    double* d_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto done = [&](int code) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (d_out) (void)hipFree(d_out);
+        return code;
+    };
+    hipError_t err;
+    if ((err = hipMalloc(&d_out, sizeof(double) * (size_t)iterations * nsys * vmc::kBatchOut)) != hipSuccess ||
+        (err = hipMemcpyAsync(b->d_table, &T, sizeof(T), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(b->d_st, &st, sizeof(st), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+        (err = hipStreamSynchronize(b->stream)) != hipSuccess)
+        return done(fail(VMC_E_DEVICE, std::string("batch run: ") + hipGetErrorString(err)));
+    auto enqueue = [&]() -> int {
+        if (int rc = batch_enqueue_walk(b, total)) return rc;
+        HIP_TRY(vmc::launch_batch_tail(b->d_table, nsys, b->d_opt, b->d_theta, b->d_acc, d_out,
+                                       b->d_st, it0, total, b->stream));
+        return VMC_OK;
+    };
+    if (use_graph) {
+        if ((err = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return done(fail(VMC_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(err)));
+        const int rc = enqueue();
+        const hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+        if (rc != VMC_OK) return done(rc);
+        if (e2 != hipSuccess) return done(fail(VMC_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(e2)));
+        if ((err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+            return done(fail(VMC_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(err)));
+    }
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    for (int it = 0; it < iterations; it++) {
+        if (use_graph) {
+            if ((err = hipGraphLaunch(exec, b->stream)) != hipSuccess)
+                return done(fail(VMC_E_DEVICE, std::string("graph launch: ") + hipGetErrorString(err)));
+        } else if (int rc = enqueue()) {
+            return done(rc);
+        }
+    }
+    std::vector<double> h((size_t)iterations * nsys * vmc::kBatchOut);
+    float ms = 0;
+    if ((err = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(h.data(), d_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                              b->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(b->theta.data(), b->d_theta, sizeof(double) * b->theta.size(),
+                              hipMemcpyDeviceToHost, b->stream)) != hipSuccess ||
+        (err = hipStreamSynchronize(b->stream)) != hipSuccess ||
+        (err = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess)
+        return done(fail(VMC_E_DEVICE, std::string("batch run: ") + hipGetErrorString(err)));
+    b->counter += (uint64_t)iterations * total;
+    b->iteration += iterations;
+    b->fresh = false;
+    if (out)
+        for (int it = 0; it < iterations; it++)
+            for (int d = 0; d < nsys; d++) {
+                const int u = b->hs[d].user;
+                for (int k = 0; k < vmc::kBatchOut; k++)
+                    out[((size_t)it * nsys + u) * vmc::kBatchOut + k] =
+                        h[((size_t)it * nsys + d) * vmc::kBatchOut + k];
+            }
+    if (kernel_ms) *kernel_ms = ms;
+    return done(VMC_OK);
+}
+
+int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, int* waves_out,
+                       double* ns_per_step_out)
+{
+    if (!b) return fail(VMC_E_INVALID, "null batch");
+    if (!b->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
+    if (steps < 1) return fail(VMC_E_INVALID, "steps >= 1");
+    const int nsys = (int)b->hs.size();
+    // each system alone in a one-entry table, walked from its saved positions
+    // and drifts, which are restored after every candidate
+    size_t maxpos = 0;
+    for (const auto& y : b->sys) maxpos = std::max(maxpos, (size_t)y.num_walkers * 2 * y.num_particles);
+    double* d_save = nullptr;
+    HIP_TRY(hipMalloc(&d_save, sizeof(double) * 2 * maxpos));
+    auto finish = [&](int code) {
+        (void)hipFree(d_save);
+        return code;
+    };
+    hipError_t err;
+    const vmc::BatchState st{b->counter, 0, 1, b->fresh ? 1 : 0, 0.0};
+    if ((err = hipMemcpy(b->d_st, &st, sizeof(st), hipMemcpyHostToDevice)) != hipSuccess)
+        return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+    for (int d = 0; d < nsys; d++) {
+        const vmc::BatchSystem B = b->hs[d];
+        const size_t n = (size_t)B.P.W * B.P.ND;
+        const size_t off = b->pos_off[B.user];
+        if ((err = hipMemcpy(d_save, b->d_pos + off, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess ||
+            (err = hipMemcpy(d_save + n, b->d_qf + off, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess)
+            return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+        double best = 1e300;
+        int bestK = B.K, bestW = B.waves;
+        for (int K : {B.K, 2 * B.K}) {
+            const int id = vmc::batch_shape_id(B.np, K);
+            if (id < 0) continue;
+            for (int waves : {1, 2, 4}) {
+                vmc::BatchTable T{};
+                T.nsys = 1;
+                vmc::BatchSystem& C = T.sys[0];
+                C = B;
+                C.K = K;
+                C.shape = id;
+                C.waves = waves;
+                C.tile0 = 0;
+                C.ntiles = batch_tiles(C);
+                C.P.seed = b->seed + (uint64_t)B.user;
+                C.P.steps = steps;
+                C.P.equil = 0;
+                T.total_tiles = C.ntiles;
+                const size_t lds = vmc::batch_lds_bytes(C.np, K);
+                const int grid = std::min(
+                    b->cus * std::max(1, vmc::batch_resident_per_cu(b->cfg.precision, lds)), C.ntiles);
+                float ms = 0;
+                for (int rep = 0; rep < 2; rep++)   // the second launch is timed
+                    if ((err = hipMemcpy(b->d_table + 1, &T, sizeof(T), hipMemcpyHostToDevice)) != hipSuccess ||
+                        (err = hipEventRecord(b->ev0, b->stream)) != hipSuccess ||
+                        (err = vmc::launch_batch_walk(b->cfg.precision, b->d_table + 1, b->d_queue, grid, lds, 0,
+                                                      steps, 0, b->d_st, b->d_theta + 2 * d,
+                                                      b->stream)) != hipSuccess ||
+                        (err = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
+                        (err = hipStreamSynchronize(b->stream)) != hipSuccess ||
+                        (err = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess ||
+                        (err = hipMemcpy(b->d_pos + off, d_save, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess ||
+                        (err = hipMemcpy(b->d_qf + off, d_save + n, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess)
+                        return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+                const double ns = 1e6 * ms / (double(C.P.W) * steps);
+                if (ns < best) {
+                    best = ns;
+                    bestK = K;
+                    bestW = waves;
+                }
+            }
+        }
+        b->hs[d].K = bestK;
+        b->hs[d].shape = vmc::batch_shape_id(B.np, bestK);
+        b->hs[d].waves = bestW;
+        if (lanes_out) lanes_out[B.user] = bestK;
+        if (waves_out) waves_out[B.user] = bestW;
+        if (ns_per_step_out) ns_per_step_out[B.user] = best;
+    }
+    batch_plan(b);
+    return finish(VMC_OK);
 }
 
 }  // extern "C"

@@ -267,6 +267,7 @@ struct Walker {
     bool valid[2];
     bool upspin[2];
     int l, gbase, g;
+    double th_alpha, th_gamma;   // Gaussian alpha and Pade gamma of the walk
 
     // RBM state (RB only).  Lane l owns hidden units h = l + K j; the sorted
     // coordinate vector of the walker and its n = sigmoid(v) live in LDS.
@@ -400,7 +401,7 @@ struct Walker {
     // Pade gradients and the walker sums, from scratch (O(N^2) per walker)
     __device__ __forceinline__ void rebuild_pairs(const WalkParams& P)
     {
-        const R gam = R(P.gamma);
+        const R gam = R(th_gamma);
         double vi = 0, ljs = 0, jgs = 0;
 #pragma unroll
         for (int s = 0; s < 2; s++) gJ[s][0] = gJ[s][1] = R(0);
@@ -450,7 +451,7 @@ struct Walker {
     // local energy of the current state (ref src/elementoperation.cpp:95-109, src/sampler.cpp:66-70)
     __device__ __forceinline__ void local_energy(const WalkParams& P)
     {
-        const R oa = R(P.omega * P.alpha);
+        const R oa = R(P.omega * th_alpha);
         double t = 0;
 #pragma unroll
         for (int s = 0; s < 2; s++) {
@@ -487,7 +488,7 @@ struct Walker {
         }
         t = gsum<K>(t);
         if (P.has_pade) t += lj;
-        if (P.has_gauss) t -= P.omega * P.alpha * 2.0 * P.N;
+        if (P.has_gauss) t -= P.omega * th_alpha * 2.0 * P.N;
         if constexpr (RB) {
             if (P.has_rbmp) t += lr;
             if (P.has_rbmg) t -= double(P.ND) / P.sig2_g;
@@ -680,8 +681,8 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
     using WK = Walker<R, NP, K, RB>;
     if (pm) k = 2 * pm->p;
     const R dt = R(P.dt), sqdt = R(::sqrt(P.dt));
-    const R oa = R(P.omega * P.alpha);
-    const R gam = R(P.gamma);
+    const R oa = R(P.omega * w.th_alpha);
+    const R gam = R(w.th_gamma);
     const int p = k >> 1, d = k & 1;
     const int slot = p >= NP ? 1 : 0;
     const int lp = p - slot * NP;
@@ -1016,7 +1017,7 @@ __device__ __forceinline__ R walker_gradient(const Walker<R, NP, K, RB>& w, cons
     R g = R(0);
     if (P.has_slater) g += w.gS[s][dd];
     if (P.has_pade) g += w.gJ[s][dd];
-    if (P.has_gauss) g -= R(P.omega * P.alpha) * w.x[s][dd];
+    if (P.has_gauss) g -= R(P.omega * w.th_alpha) * w.x[s][dd];
     if constexpr (RB) {
         if (P.has_rbmp) g += w.gR[s][dd];
         if (P.has_rbmg) g -= (w.x[s][dd] - w.ra[s][dd]) * R(1.0 / P.sig2_g);
@@ -1272,25 +1273,38 @@ __device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& 
 // ------------------------------------------------------------------------
 // the walk kernel: equil + steps Metropolis steps per walker
 // ------------------------------------------------------------------------
+// LDS the walk of one block needs besides the dynamic RBM / learned-proposal
+// area: the Gauss-Jordan transpose and pivot rows of the Slater rebuild
+__device__ __forceinline__ WalkDyn walk_dyn(const WalkParams& P)
+{
+    return WalkDyn{P.alpha,   P.gamma,     P.counter0,  P.t_begin, P.t_end,
+                   P.acc_add, P.window_lo, P.window_hi, P.init_qf};
+}
+
+template <int NP, int K>
+constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 2 * (64 / K) * NP; }
+
+// the walk of the G = 64/K walkers of block `block` (one wave); lds_T and
+// lds_row as walk_static_lds_doubles, rbm_dyn the dynamic area
 template <typename R, int NP, int K, bool RB, bool LP>
-__global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
+__device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D, int block,
+                                           double* lds_T, double* lds_row, double* rbm_dyn)
 {
     using WK = Walker<R, NP, K, RB>;
     constexpr int G = WK::G;
-    __shared__ double lds_T[G * NP * NP];
-    __shared__ double lds_row[2 * G * NP];
-    extern __shared__ double rbm_dyn[];
 
     WK w;
+    w.th_alpha = D.alpha;
+    w.th_gamma = D.gamma;
     const int lane = threadIdx.x;
     w.l = lane & (K - 1);
     w.gbase = lane & ~(K - 1);
     w.g = lane / K;
-    const int walker_raw = blockIdx.x * G + w.g;
+    const int walker_raw = block * G + w.g;
     const bool live = walker_raw < P.W;
     const int walker = live ? walker_raw : P.W - 1;
     const int nd = 2 * P.N;
-    const int glive = (P.W - (int)blockIdx.x * G) < G ? (P.W - (int)blockIdx.x * G) : G;
+    const int glive = (P.W - block * G) < G ? (P.W - block * G) : G;
 
     RbmLds<R, K> L;
     LpLds<R, K> LL;
@@ -1312,7 +1326,7 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
     if (P.has_slater) w.rebuild_slater(P, lds_T, lds_row);
     w.rebuild_pairs(P);
     w.local_energy(P);
-    if (P.init_qf && P.t_begin == 0) {
+    if (D.init_qf && D.t_begin == 0) {
 #pragma unroll
         for (int s = 0; s < 2; s++) {
             w.qf[s][0] = R(2) * walker_gradient(w, P, s, 0);
@@ -1332,10 +1346,10 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
     int rk = 0;
     R rz = R(0), rum = R(0), rua = R(0), rz2 = R(0);
 
-    for (int t = P.t_begin; t < P.t_end; t++) {
+    for (int t = D.t_begin; t < D.t_end; t++) {
         const int tk = t & (K - 1);
-        if (tk == 0 || t == P.t_begin)
-            step_draws<R>(P.seed, (uint32_t)walker, P.counter0 + (uint64_t)(t - tk + w.l), nd, rk,
+        if (tk == 0 || t == D.t_begin)
+            step_draws<R>(P.seed, (uint32_t)walker, D.counter0 + (uint64_t)(t - tk + w.l), nd, rk,
                           rz, rum, rua, rz2);
         const int src = w.gbase + tk;
         const int k = __shfl(rk, src, 64);
@@ -1367,7 +1381,7 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
             aV2 += w.vext * w.vext;
             aI2 += w.vint * w.vint;
             aAcc += accept ? 1.0 : 0.0;
-            if (t > P.window_lo && t < P.window_hi) {      // ref sampler.cpp:82-86
+            if (t > D.window_lo && t < D.window_hi) {      // ref sampler.cpp:82-86
                 aGa += w.ga;
                 aGg += w.gg;
                 aGaE += w.ga * E;
@@ -1463,7 +1477,7 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
         }
         const int half = P.ND + P.H + P.ND * P.H;
         const double wscale = P.q6_w_unscaled ? 1.0 : P.isig2_p;
-        double* row = P.rbm_acc + (size_t)blockIdx.x * rbm_row_width(P.ND, P.H);
+        double* row = P.rbm_acc + (size_t)block * rbm_row_width(P.ND, P.H);
         for (int i = lane; i < 2 * half; i += 64) {
             const int hf = i >= half ? 1 : 0;
             const int r = i - hf * half;
@@ -1474,7 +1488,7 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
                 const int m = r - nab, kc = m % P.ND, h = m / P.ND;
                 v = L.accW[hf * P.ND * P.H + kc * P.H + h] * wscale;
             }
-            row[i] = P.acc_add ? row[i] + v : v;
+            row[i] = D.acc_add ? row[i] + v : v;
         }
     }
 
@@ -1497,8 +1511,17 @@ __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
         const double v[kNumWalkerAcc] = {aE, aE2, aT, aV, aI, aT2, aV2, aI2, aAcc,
                                          aGa, aGg, aGaE, aGgE, double(bad)};
 #pragma unroll
-        for (int i = 0; i < kNumWalkerAcc; i++) a[i] = P.acc_add ? a[i] + v[i] : v[i];
+        for (int i = 0; i < kNumWalkerAcc; i++) a[i] = D.acc_add ? a[i] + v[i] : v[i];
     }
+}
+
+template <typename R, int NP, int K, bool RB, bool LP>
+__global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
+{
+    __shared__ double lds[walk_static_lds_doubles<NP, K>()];
+    extern __shared__ double rbm_dyn[];
+    walk_block<R, NP, K, RB, LP>(P, walk_dyn(P), blockIdx.x, lds, lds + (64 / K) * NP * NP,
+                                 rbm_dyn);
 }
 
 // ------------------------------------------------------------------------
@@ -1551,6 +1574,8 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
     __shared__ double lds_row[2 * G * NP];
     extern __shared__ double rbm_dyn[];
     WK w;
+    w.th_alpha = P.alpha;
+    w.th_gamma = P.gamma;
     const int lane = threadIdx.x;
     w.l = lane & (K - 1);
     w.gbase = lane & ~(K - 1);

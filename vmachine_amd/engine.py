@@ -51,6 +51,22 @@ class VmcError(RuntimeError):
     pass
 
 
+class VmcBatchSystem(ctypes.Structure):
+    _fields_ = [("num_particles", _I), ("omega", _D), ("num_walkers", _I),
+                ("step_length", _D), ("alpha", _D), ("gamma", _D)]
+
+
+class VmcBatchConfig(ctypes.Structure):
+    _fields_ = [("num_systems", _I), ("systems", ctypes.POINTER(VmcBatchSystem)),
+                ("num_elements", _I), ("kinds", _I * 6), ("interaction", _I), ("sampling", _I),
+                ("quirks", _U), ("precision", _I), ("refresh_interval", _I),
+                ("drift_limit", _D)]
+
+
+BATCH_OUT = 16   # per system and iteration: the 14 walk sums (VMC_EVAL order), alpha, gamma
+BATCH_MAX_SYSTEMS = 8
+
+
 class LpHyper(ctypes.Structure):
     _fields_ = [("sigma0", _D), ("mu_max", _D), ("ls_max", _D)]
 
@@ -98,6 +114,17 @@ def lib():
         L.vmc_lp_get.argtypes = [ctypes.c_void_p, _FP, _I]
         L.vmc_lp_gradient.argtypes = [ctypes.c_void_p, _DP, _I, _IP, _DP, _DP, _DP]
         L.vmc_lp_train.argtypes = [ctypes.c_void_p, _I, _I, _D, _DP]
+        L.vmc_batch_create.argtypes = [ctypes.POINTER(VmcBatchConfig),
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        L.vmc_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.vmc_batch_destroy.restype = None
+        L.vmc_batch_init_walkers.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        for n in ["vmc_batch_set_positions", "vmc_batch_get_positions"]:
+            getattr(L, n).argtypes = [ctypes.c_void_p, _I, _DP, _I, _I]
+        L.vmc_batch_get_parameters.argtypes = [ctypes.c_void_p, _DP]
+        L.vmc_batch_set_shape.argtypes = [ctypes.c_void_p, _I, _I, _I]
+        L.vmc_batch_autotune.argtypes = [ctypes.c_void_p, _I, _IP, _IP, _DP]
+        L.vmc_batch_run.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, _D, _I, _DP, _DP]
         _lib = L
     return _lib
 
@@ -292,3 +319,104 @@ class Engine:
                                                  _dp(g1)))
         mp = self.maxpar
         return o0, o1, r, g0[:, :, :mp], g1[:, :, :mp]
+
+
+class Batch:
+    """Several systems walked by one persistent launch (vmc_batch_* in include/vmc.h).
+
+    systems: list of dicts with num_particles, omega, num_walkers, step_length,
+    alpha, gamma.  System s draws from seed + s and retraces an Engine of that
+    system seeded seed + s."""
+
+    def __init__(self, systems, kinds=(SLATER, GAUSSIAN, PADE), interaction=True,
+                 sampling=IMPORTANCE_EXACT, quirks=QUIRKS_REFERENCE, precision=64,
+                 refresh_interval=0, drift_limit=0.0):
+        self.L = lib()
+        self.n = len(systems)
+        self._sys = (VmcBatchSystem * self.n)()
+        for i, y in enumerate(systems):
+            for k in ("num_particles", "omega", "num_walkers", "step_length", "alpha", "gamma"):
+                setattr(self._sys[i], k, y[k])
+        self.systems = [dict(y) for y in systems]
+        c = VmcBatchConfig()
+        c.num_systems = self.n
+        c.systems = ctypes.cast(self._sys, ctypes.POINTER(VmcBatchSystem))
+        c.num_elements = len(kinds)
+        for i, k in enumerate(kinds):
+            c.kinds[i] = k
+        c.interaction = int(bool(interaction))
+        c.sampling = sampling
+        c.quirks = quirks
+        c.precision = precision
+        c.refresh_interval = refresh_interval
+        c.drift_limit = drift_limit
+        self.cfg = c
+        h = ctypes.c_void_p()
+        self._check(self.L.vmc_batch_create(ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VmcError(f"vmc error {rc}: {self.L.vmc_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init_walkers(self, seed):
+        self._check(self.L.vmc_batch_init_walkers(self.h, seed))
+
+    def _shape(self, s):
+        y = self.systems[s]
+        return y["num_walkers"], 2 * y["num_particles"]
+
+    def set_positions(self, s, pos):
+        W, nd = self._shape(s)
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(W, nd)
+        self._check(self.L.vmc_batch_set_positions(self.h, s, _dp(p), W, nd))
+
+    def get_positions(self, s):
+        W, nd = self._shape(s)
+        p = np.zeros((W, nd))
+        self._check(self.L.vmc_batch_get_positions(self.h, s, _dp(p), W, nd))
+        return p
+
+    def parameters(self):
+        """[num_systems][2]: alpha, gamma after the last run."""
+        ag = np.zeros((self.n, 2))
+        self._check(self.L.vmc_batch_get_parameters(self.h, _dp(ag)))
+        return ag
+
+    def set_shape(self, s, lanes_per_walker, waves_per_tile):
+        self._check(self.L.vmc_batch_set_shape(self.h, s, lanes_per_walker, waves_per_tile))
+
+    def autotune(self, steps):
+        lanes = np.zeros(self.n, dtype=np.int32)
+        waves = np.zeros(self.n, dtype=np.int32)
+        ns = np.zeros(self.n)
+        self._check(self.L.vmc_batch_autotune(self.h, steps, lanes.ctypes.data_as(_IP),
+                                              waves.ctypes.data_as(_IP), _dp(ns)))
+        return lanes, waves, ns
+
+    def run(self, iterations, steps, equilibration=0, batches=1, learning_rate=0.0,
+            use_graph=True):
+        """(out [iterations][num_systems][BATCH_OUT], device milliseconds)."""
+        out = np.zeros((iterations, self.n, BATCH_OUT))
+        ms = ctypes.c_double(0)
+        self._check(self.L.vmc_batch_run(self.h, iterations, steps, equilibration, batches,
+                                         learning_rate, int(bool(use_graph)), _dp(out),
+                                         ctypes.byref(ms)))
+        return out, ms.value
