@@ -84,6 +84,14 @@ constexpr int max_degree(int np) { return np <= 1 ? 0 : orb_shell(np - 1); }
 template <typename R> struct M;
 template <> struct M<float> {
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+    // the hardware square root and base-2 logarithm (v_sqrt_f32, v_log_f32, about
+    // 1 ulp) for the per-step distances, drift limits and acceptance tests; the
+    // correctly rounded forms above stay in the draws and the rebuilds
+    static __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float flog(float x)
+    {
+        return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
+    }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float exp(float x) { return __expf(x); }
     static __device__ __forceinline__ float log(float x) { return __logf(x); }
@@ -92,6 +100,8 @@ template <> struct M<float> {
 };
 template <> struct M<double> {
     static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double fsqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double flog(double x) { return ::log(x); }
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
     static __device__ __forceinline__ double log(double x) { return ::log(x); }
@@ -452,7 +462,10 @@ struct Walker {
     __device__ __forceinline__ void local_energy(const WalkParams& P)
     {
         const R oa = R(P.omega * th_alpha);
-        double t = 0;
+        // the lane terms and their group sum in the walk's precision (FP32
+        // values of O(1) each: the sum over 28 lanes keeps 1e-7 relative);
+        // the walker totals and the accumulation stay FP64
+        R t = R(0);
 #pragma unroll
         for (int s = 0; s < 2; s++) {
             if (!valid[s]) continue;
@@ -480,20 +493,18 @@ struct Walker {
                     o1 -= (x[s][1] - ra[s][1]) * is2;
                 }
             }
-            double v = double(o0) * double(o0) + double(o1) * double(o1);
-            if (P.has_slater)
-                v += double(sS[s]) + 2.0 * (double(gS[s][0]) * double(o0) +
-                                            double(gS[s][1]) * double(o1));
+            R v = o0 * o0 + o1 * o1;
+            if (P.has_slater) v += sS[s] + R(2) * (gS[s][0] * o0 + gS[s][1] * o1);
             t += v;
         }
-        t = gsum<K>(t);
-        if (P.has_pade) t += lj;
-        if (P.has_gauss) t -= P.omega * th_alpha * 2.0 * P.N;
+        double tt = double(gsum<K>(t));
+        if (P.has_pade) tt += lj;
+        if (P.has_gauss) tt -= P.omega * th_alpha * 2.0 * P.N;
         if constexpr (RB) {
-            if (P.has_rbmp) t += lr;
-            if (P.has_rbmg) t -= double(P.ND) / P.sig2_g;
+            if (P.has_rbmp) tt += lr;
+            if (P.has_rbmg) tt -= double(P.ND) / P.sig2_g;
         }
-        tk = -0.5 * t;
+        tk = -0.5 * tt;
         vext = 0.5 * P.omega * P.omega * sumx2;
         el = tk + vext + vint;
         ga = P.has_gauss ? -0.5 * P.omega * sumx2 : 0.0;
@@ -717,7 +728,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
     // limit a > 0, 2F / (1 + sqrt(1 + a F^2 dt / 2))
     const bool limit = P.sampling == kImportanceExact && P.drift_a > 0;
     const R Fp =
-        limit ? R(2) * F * M<R>::rcp(R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * F * F * dt)) : F;
+        limit ? R(2) * F * M<R>::rcp(R(1) + M<R>::fsqrt(R(1) + R(0.5 * P.drift_a) * F * F * dt)) : F;
     R dx;
     if (forced) dx = xforced - xpd;
     else if (P.sampling == kBruteForce) dx = R(10) * (um - R(0.5)) * dt;
@@ -735,8 +746,8 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         const bool val = w.valid[s] && q != p;
         const R e0 = w.x[s][0] - xp0, e1 = w.x[s][1] - xp1;
         const R n0 = w.x[s][0] - xn0, n1 = w.x[s][1] - xn1;
-        const R ro = M<R>::sqrt(e0 * e0 + e1 * e1);
-        const R rn = M<R>::sqrt(n0 * n0 + n1 * n1);
+        const R ro = M<R>::fsqrt(e0 * e0 + e1 * e1);
+        const R rn = M<R>::fsqrt(n0 * n0 + n1 * n1);
         const R iro = val ? M<R>::rcp(ro) : R(0);
         const R irn = val ? M<R>::rcp(rn) : R(0);
         const R a = (w.upspin[s] == pup) ? R(P.a_same) : R(P.a_anti);
@@ -769,11 +780,21 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         // no walker of the wave needs it)
         const bool need = slot != w.act;
         if (__any(need)) {
+            if constexpr (std::is_same<R, float>::value) {
+                // one register swap per entry on the lanes that need it
+                // (exec-masked) instead of two selects per entry
+                if (need) {
 #pragma unroll
-            for (int kk = 0; kk < NP; kk++) {
-                const R t0 = w.a0[kk], t1 = w.a1[kk];
-                w.a0[kk] = need ? t1 : t0;
-                w.a1[kk] = need ? t0 : t1;
+                    for (int kk = 0; kk < NP; kk++)
+                        asm("v_swap_b32 %0, %1" : "+v"(w.a0[kk]), "+v"(w.a1[kk]));
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < NP; kk++) {
+                    const R t0 = w.a0[kk], t1 = w.a1[kk];
+                    w.a0[kk] = need ? t1 : t0;
+                    w.a1[kk] = need ? t0 : t1;
+                }
             }
             w.act = slot;
         }
@@ -883,19 +904,19 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
     // exponent by tens, and the product of the factors would leave the FP32 range
     auto log_ratio = [&]() {
         R lr = R(0);
-        if (P.has_slater) lr += R(2) * M<R>::log(Rs < R(0) ? -Rs : Rs);
+        if (P.has_slater) lr += R(2) * M<R>::flog(Rs < R(0) ? -Rs : Rs);
         if (P.has_pade) lr += R(2) * jexp;
         if (P.has_gauss)
             lr += pm ? oa * (xp0 * xp0 + xp1 * xp1 - xn0 * xn0 - xn1 * xn1)
                      : oa * (xpd * xpd - xnd * xnd);
-        if constexpr (RB) lr += M<R>::log(rrg) + M<R>::log(rrp);
+        if constexpr (RB) lr += M<R>::flog(rrg) + M<R>::flog(rrp);
         return lr;
     };
     bool accept;
     if (forced) {
         accept = true;
     } else if (pm) {
-        accept = log_ratio() + pm->logq >= M<R>::log(ua);
+        accept = log_ratio() + pm->logq >= M<R>::flog(ua);
     } else if (P.sampling == kBruteForce) {
         accept = !(ratio < ua);
     } else if (P.sampling == kImportanceCompat) {
@@ -907,8 +928,8 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         } else {
             // FP32: the same test in the log domain, log w = softplus(xg) + log ratio,
             // which neither overflows nor forms inf x 0 next to a node
-            const R sp = (xg > R(0) ? xg : R(0)) + M<R>::log(R(1) + M<R>::exp(-(xg > R(0) ? xg : -xg)));
-            accept = sp + log_ratio() >= M<R>::log(ua);
+            const R sp = (xg > R(0) ? xg : R(0)) + M<R>::flog(R(1) + M<R>::exp(-(xg > R(0) ? xg : -xg)));
+            accept = sp + log_ratio() >= M<R>::flog(ua);
         }
     } else {
         R Gn = R(0);
@@ -921,14 +942,14 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         }
         const R Fr = R(2) * Gn;
         const R Fn =
-            limit ? R(2) * Fr * M<R>::rcp(R(1) + M<R>::sqrt(R(1) + R(0.5 * P.drift_a) * Fr * Fr * dt))
+            limit ? R(2) * Fr * M<R>::rcp(R(1) + M<R>::fsqrt(R(1) + R(0.5 * P.drift_a) * Fr * Fr * dt))
                   : Fr;
         const R lg = R(-0.5) * dx * (Fp + Fn) + dt * R(0.125) * (Fp * Fp - Fn * Fn);
         // an undefined weight (0 x inf next to a node) is rejected
         if constexpr (std::is_same<R, double>::value)
             accept = ratio * M<R>::exp(lg) >= ua;
         else
-            accept = log_ratio() + lg >= M<R>::log(ua);
+            accept = log_ratio() + lg >= M<R>::flog(ua);
     }
     if (!accept) return false;
 
