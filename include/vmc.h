@@ -141,6 +141,32 @@ int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration,
                      vmc_sums* sums, double* grad, double* grad_e, int record_walkers,
                      float* energies);
 
+/* As vmc_run_recorded with all four instant values of the first
+ * `record_walkers` walkers: out [4][record_walkers][steps] (FP32) = local
+ * energy, kinetic, external, interaction, the four series
+ * Sampler::printInstantValuesToFile writes (ref src/sampler.cpp:624-632). */
+int vmc_run_instant(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+                    vmc_sums* sums, double* grad, double* grad_e, int record_walkers, float* out);
+
+/* Electron densities, counted by the walk after every sampled step of every
+ * walker while enabled (ref Sampler::computeOneBodyDensity,
+ * computeOneBodyDensity2, computeTwoBodyDensity, src/sampler.cpp:639-692),
+ * with dr = max_radius / bins and r_i = |x_i|:
+ *   VMC_DENSITY_RADIAL: radial [bins], r < max_radius at int(r/dr);
+ *   VMC_DENSITY_SPATIAL: spatial [bins][bins] (row = y), cell
+ *     (int(y/2dr) + bins/2, int(x/2dr) + bins/2) with C truncation, both
+ *     |int(.)| < bins/2;
+ *   VMC_DENSITY_TWO_BODY: two_body [bins][bins], every pair i < j at
+ *     (int(r_i/dr) + 1, int(r_j/dr) + 1), both < bins.
+ * vmc_set_density clears the counts (flags 0 switches them off);
+ * vmc_get_density copies them out (any pointer may be null). */
+#define VMC_DENSITY_RADIAL 1
+#define VMC_DENSITY_SPATIAL 2
+#define VMC_DENSITY_TWO_BODY 4
+#define VMC_DENSITY_MAX_BINS 4096
+int vmc_set_density(vmc_engine* e, int flags, int bins, double max_radius);
+int vmc_get_density(vmc_engine* e, int64_t* radial, int64_t* spatial, int64_t* two_body);
+
 /* Parity entry point.  For each of `walkers` configurations (positions
  * row-major [walkers][dof]) the engine rebuilds its state, evaluates the local
  * energy and its parts, dlnpsi/dtheta and the drift, then forces the move of

@@ -159,6 +159,9 @@ def lib():
         L.vmco_lp_step.restype = _I
         L.vmco_lp_sample_grad.argtypes = [_P, _DP, ctypes.POINTER(LpHyper), _I, _D, _D, _DP, _DP]
         L.vmco_lp_sample_grad.restype = _D
+        _I64P = ctypes.POINTER(ctypes.c_int64)
+        L.vmco_density_add.argtypes = [_DP, _I, _I, _D, _I64P, _I64P, _I64P]
+        L.vmco_density_add.restype = None
         _lib = L
     return _lib
 
@@ -355,6 +358,17 @@ def blocking(x):
     if rc != 0:
         raise ValueError("blocking needs a power-of-two length")
     return mean.value, se.value, nk.value
+
+
+def density_add(x, N, bins, max_radius, radial=None, spatial=None, two_body=None):
+    """Adds one configuration's counts to int64 histograms (radial [bins],
+    spatial and two_body [bins, bins]) in place."""
+    P = ctypes.POINTER(ctypes.c_int64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    ptr = [h.ctypes.data_as(P) if h is not None else None for h in (radial, spatial, two_body)]
+    for h in (radial, spatial, two_body):
+        assert h is None or (h.dtype == np.int64 and h.flags.c_contiguous)
+    lib().vmco_density_add(_dp(xx), N, bins, max_radius, *ptr)
 
 
 def chi2_quantile95(dof):

@@ -1376,3 +1376,45 @@ double vmco_lp_sample_grad(vmco_walker* w, const double* th, const vmco_lp_hyper
     }
     return J;
 }
+
+/* ---- electron densities ----
+ * r_i = sqrt(sum_d x_id^2) (ref Metropolis::calculateRadialVectorElement,
+ * src/Metropolis/metropolis.cpp:31-39), dr = maxRadius / numberOfBins (ref
+ * src/sampler.cpp:41). */
+void vmco_density_add(const double* x, int N, int bins, double max_radius, int64_t* radial,
+                      int64_t* spatial, int64_t* two_body)
+{
+    const double dr = max_radius / bins;
+    const int half = bins / 2;   /* ref sampler.cpp:35 */
+    for (int i = 0; i < N; i++) {
+        const double ri = sqrt(x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1]);
+        /* radial one-body (ref sampler.cpp:639-652) */
+        if (radial && ri < max_radius) {
+            const int bin = (int)(ri / dr);
+            if (bin < bins) radial[bin]++;
+        }
+        /* spatial one-body (ref sampler.cpp:658-673): indices by truncation of
+         * x_d / (2 dr), row = y index + half, column = x index + half */
+        if (spatial) {
+            const double fx = x[2 * i] / (2 * dr), fy = x[2 * i + 1] / (2 * dr);
+            if (fabs(fx) < 2.0e9 && fabs(fy) < 2.0e9) {
+                const int ix = (int)fx, iy = (int)fy;
+                if (abs(ix) < half && abs(iy) < half)
+                    spatial[(int64_t)(iy + half) * bins + (ix + half)]++;
+            }
+        }
+        /* radial two-body (ref sampler.cpp:679-692): bins shifted by one */
+        if (two_body) {
+            const double qi = ri / dr;
+            if (qi >= 2.0e9) continue;
+            const int bin_i = (int)qi + 1;
+            for (int j = i + 1; j < N; j++) {
+                const double rj = sqrt(x[2 * j] * x[2 * j] + x[2 * j + 1] * x[2 * j + 1]);
+                const double qj = rj / dr;
+                if (qj >= 2.0e9) continue;
+                const int bin_j = (int)qj + 1;
+                if (bin_i < bins && bin_j < bins) two_body[(int64_t)bin_i * bins + bin_j]++;
+            }
+        }
+    }
+}

@@ -206,6 +206,13 @@ double vmco_lp_sample_grad(vmco_walker* w, const double* theta, const vmco_lp_hy
                            double xi0, double xi1, double* grad,
                            double* out /* 6: J, A, logR, x', y', L */);
 
+/* ---- Electron densities (src/sampler.cpp:639-692) ----
+ * Add the counts of one configuration x [N*2] to the histograms (any may be
+ * NULL): radial [bins], spatial [bins][bins] (row = y cell), two_body
+ * [bins][bins]; dr = max_radius / bins. */
+void vmco_density_add(const double* x, int N, int bins, double max_radius, int64_t* radial,
+                      int64_t* spatial, int64_t* two_body);
+
 int vmco_num_orbitals(int N, int D);  /* n = N/2 for closed shells, -1 if open */
 void vmco_orbital_list(int n, int D, int* states /* n*D */);
 /* value / derivative / second derivative of orbital j along dimension d at r */

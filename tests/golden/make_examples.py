@@ -4,9 +4,10 @@ Run in the build container only (it reads the reference tree):
     python tests/golden/make_examples.py
 
 For each directory under the reference's examples/ it stores the
-configuration's key lines (comments and blank lines dropped) as <name>.in, and
-the numbers of the shipped run log (examples/<name>/vmachine.log) in
-expected.json: the per-iteration table and the final block.  The logs are the
+configuration's key lines (comments and blank lines dropped) as <name>.in, the
+numbers of the shipped run log (examples/<name>/vmachine.log) in
+expected.json (the per-iteration table and the final block), and the density
+files the example ships (<name>/onebody2.dat etc.) verbatim.  The logs are the
 reference's own outputs for those inputs; the engine's `vmachine` runs are
 compared against them statistically (different random streams).
 """
@@ -18,6 +19,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "examples")
 SRC = "/root/reference/examples"
+
+DATA_FILES = {"onebody.dat", "onebody2.dat", "twobody.dat"}
 
 FINAL = {
     "Total energy": "energy",
@@ -77,6 +80,14 @@ def main():
         with open(log, encoding="utf-8", errors="replace") as f:
             rows, final = parse_log(f.read())
         expected[name] = {"rows": rows, "final": final}
+        # output data files the example ships beside its log (e.g. the spatial
+        # density onebody2.dat), verbatim
+        for fn in sorted(os.listdir(d)):
+            if fn.endswith(".dat") and fn in DATA_FILES:
+                os.makedirs(os.path.join(OUT, name), exist_ok=True)
+                with open(os.path.join(d, fn), "rb") as src, \
+                        open(os.path.join(OUT, name, fn), "wb") as dst:
+                    dst.write(src.read())
     with open(os.path.join(OUT, "expected.json"), "w") as f:
         json.dump(expected, f, indent=1)
     print("wrote", len(expected), "examples to", OUT)

@@ -62,10 +62,41 @@ def test_no_interaction_example_is_exact():
     assert final["interaction"] == 0.0
 
 
+def _grid_r2(g, dr2):
+    """Mean r^2 of the counts of a spatial grid (cell centres; the centre row
+    and column span two cells' width around 0)."""
+    n = g.shape[0]
+    half = n // 2
+    idx = np.arange(n) - half
+    centre = np.where(idx == 0, 0.0, (idx + np.sign(idx) * 0.5) * dr2)
+    # cells left of the centre: index i < 0 covers ((i-1) dr2, i dr2]; right: [i dr2, (i+1) dr2)
+    centre = np.where(idx < 0, (idx - 0.5) * dr2, centre)
+    X, Y = np.meshgrid(centre, centre)
+    return float((g * (X * X + Y * Y)).sum() / g.sum())
+
+
 @pytest.mark.parametrize("name", ["small_fermionic_quantumdot_vmc", "electron_density"])
-def test_two_electron_dot_examples(name):
-    rows, final, _ = _run(name, "--seed", 12)
+def test_two_electron_dot_examples(name, tmp_path):
+    rows, final, _ = _run(name, "--seed", 12, cwd=tmp_path)
     _compare(name, rows, final)
+    if name == "electron_density":
+        # the spatial one-body density of the last iteration (maxRadius 3, 100
+        # bins) against the reference's own onebody2.dat: Eigen's integer
+        # format, the centre row and column spanning 4 dr, N x (sampled steps)
+        # counts, and the same <r^2> within 0.05 (statistical error ~0.012)
+        ref = np.loadtxt(os.path.join(EX, "electron_density", "onebody2.dat"), dtype=np.int64)
+        text = (tmp_path / "onebody2.dat").read_text()
+        lines = text.split("\n")
+        assert lines[-1] == "" and len(lines) == 101
+        g = np.array([[int(v) for v in ln.split()] for ln in lines[:-1]], dtype=np.int64)
+        assert g.shape == (100, 100)
+        w = len(str(g.max()))
+        assert all(len(ln) == 100 * w + 99 for ln in lines[:-1])
+        assert 0.99 * 200000 < g.sum() <= 200000
+        assert g[50, 50] == g.max()
+        r2, r2ref = _grid_r2(g, 0.06), _grid_r2(ref, 0.06)
+        print(f"  <r^2> from onebody2.dat: {r2:.4f} (reference {r2ref:.4f})")
+        assert abs(r2 - r2ref) < 0.05
 
 
 def test_twelve_electron_dot_example():
@@ -79,18 +110,49 @@ def test_rbm_example():
     _compare("small_fermionic_quantumdot_rbm", rows, final)
 
 
-def test_resampling_example_blocks_the_last_iteration():
-    rows, final, out = _run("resampling", "--seed", 14)
+def test_resampling_example_blocks_the_last_iteration(tmp_path):
+    rows, final, out = _run("resampling", "--seed", 14, cwd=tmp_path)
     _compare("resampling", rows, final)
-    # the final standard error comes from blocking the recorded local energies:
+    # the final standard errors come from blocking the recorded instant values:
     # correlated samples give an error above the naive sqrt(var/M) of the rows
     ref = EXPECTED["resampling"]["final"]
-    assert 0.2 * ref["energy_std"] < final["energy_std"] < 5 * ref["energy_std"]
+    for k in ("energy", "kinetic", "external", "interaction"):
+        assert 0.2 * ref[k + "_std"] < final[k + "_std"] < 5 * ref[k + "_std"], k
+        assert abs(final[k] - ref[k]) < 5 * (final[k + "_std"] + ref[k + "_std"]), k
+    # the instant-energy file <n>_0.dat stays, the three part files are removed
+    files = sorted(p.name for p in tmp_path.iterdir())
+    inst = [f for f in files if f.endswith("_0.dat")]
+    assert len(inst) == 1 and not any(f.endswith(("_kin.dat", "_ext.dat", "_int.dat")) for f in files)
+    e = np.loadtxt(tmp_path / inst[0])
+    assert e.size >= 1024 and abs(e.mean() - final["energy"]) < 0.01
 
 
-def test_fp32_walk_matches_fp64_statistically():
-    r64, f64, _ = _run("small_fermionic_quantumdot_vmc", "--seed", 15)
-    r32, f32, _ = _run("small_fermionic_quantumdot_vmc", "--seed", 15, "--precision", 32)
+def test_density_files(tmp_path):
+    """Radial one-body and two-body densities of a short run: Eigen's integer
+    format, N x (sampled steps) radial counts inside maxRadius, the two-body
+    bins shifted by one (row and column 0 empty)."""
+    cfg = tmp_path / "input.in"
+    text = open(os.path.join(EX, "small_fermionic_quantumdot_vmc.in")).read()
+    text = text.replace("numIterations: 100", "numIterations: 6")
+    cfg.write_text(text + "computeOneBodyDensity: true\ncomputeTwoBodyDensity: true\n"
+                          "maxRadius: 4\nnumberOfBins: 50\n")
+    print("\n[vmachine] density files", flush=True)
+    r = H.run(["--seed", 18, str(cfg)], timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    one = np.loadtxt(tmp_path / "onebody.dat", dtype=np.int64)
+    two = np.loadtxt(tmp_path / "twobody.dat", dtype=np.int64)
+    assert one.shape == (50,) and two.shape == (50, 50)
+    assert two[0, :].sum() == 0 and two[:, 0].sum() == 0
+    w = len(str(one.max()))
+    assert all(len(ln) == w for ln in (tmp_path / "onebody.dat").read_text().split("\n")[:-1])
+    # 2 particles: one pair per sampled step, both radii below 49 dr
+    assert 0.9 * one.sum() / 2 < two.sum() <= one.sum() / 2 + 1
+
+
+def test_fp32_walk_matches_fp64_statistically(tmp_path):
+    r64, f64, _ = _run("small_fermionic_quantumdot_vmc", "--seed", 15, cwd=tmp_path)
+    r32, f32, _ = _run("small_fermionic_quantumdot_vmc", "--seed", 15, "--precision", 32,
+                       cwd=tmp_path)
     tol = 3 * (r64[-20:, 1].std() + r32[-20:, 1].std())
     assert abs(r64[-20:, 1].mean() - r32[-20:, 1].mean()) < tol
 

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <ctime>
 #include <fstream>
 #include <iomanip>
@@ -122,6 +124,24 @@ struct Averages {
 
 }  // namespace
 
+// an integer matrix as the reference writes it, Eigen's default stream format:
+// every coefficient right-aligned to the widest one, one space between
+// columns, one row per line (ref tests/Eigen/src/Core/IO.h:129-199)
+static void write_counts(const std::string& path, const std::vector<int64_t>& v, int rows,
+                         int cols)
+{
+    size_t w = 1;
+    for (int64_t x : v) w = std::max(w, std::to_string(x).size());
+    std::ofstream f(path);
+    for (int i = 0; i < rows; i++) {
+        for (int j = 0; j < cols; j++) {
+            if (j) f << ' ';
+            f << std::setw((int)w) << v[(size_t)i * cols + j];
+        }
+        f << '\n';
+    }
+}
+
 bool blocking(std::vector<double> x, double& mean, double& stderr_)
 {
     const size_t n = x.size();
@@ -218,8 +238,15 @@ int run_simulation(const Config& c, const RunOptions& o, std::ostream& out, std:
         err << "The engine draws initialState: randomNormal" << std::endl;
         return 143;
     }
-    if (c.computeOneBodyDensity || c.computeOneBodyDensity2 || c.computeTwoBodyDensity)
-        err << "note: electron densities are not computed by the engine" << std::endl;
+    const int dens_flags = (c.computeOneBodyDensity ? VMC_DENSITY_RADIAL : 0) |
+                           (c.computeOneBodyDensity2 ? VMC_DENSITY_SPATIAL : 0) |
+                           (c.computeTwoBodyDensity ? VMC_DENSITY_TWO_BODY : 0);
+    if (dens_flags && (c.numberOfBins < 1 || c.numberOfBins > VMC_DENSITY_MAX_BINS ||
+                       !(c.maxRadius > 0))) {
+        err << "electron densities need numberOfBins in 1.." << VMC_DENSITY_MAX_BINS
+            << " and maxRadius > 0" << std::endl;
+        return 143;
+    }
     vmc_config ec = {};
     ec.num_particles = c.numParticles;
     ec.num_dimensions = c.numDimensions;
@@ -353,11 +380,16 @@ int run_simulation(const Config& c, const RunOptions& o, std::ostream& out, std:
         vmc_sums s;
         std::vector<double> g((size_t)rows * std::max(cols, 1)), ge(g.size());
         int rc;
+        // the last iteration counts the densities (ref src/system.cpp:155-161)
+        if (last && dens_flags &&
+            vmc_set_density(eng, dens_flags, c.numberOfBins, c.maxRadius) != VMC_OK)
+            return fail("densities");
+        int rw = 0;
         if (last && c.doResampling) {
             const long long cap = 1LL << 24;
-            const int rw = (int)std::max(1LL, std::min<long long>(W, cap / S));
-            rec.assign((size_t)rw * S, 0.f);
-            rc = vmc_run_recorded(eng, S, equil, printed, B, &s, g.data(), ge.data(), rw, rec.data());
+            rw = (int)std::max(1LL, std::min<long long>(W, cap / S));
+            rec.assign((size_t)4 * rw * S, 0.f);
+            rc = vmc_run_instant(eng, S, equil, printed, B, &s, g.data(), ge.data(), rw, rec.data());
         } else {
             rc = vmc_run(eng, S, equil, printed, B, &s, g.data(), ge.data());
         }
@@ -414,25 +446,62 @@ int run_simulation(const Config& c, const RunOptions& o, std::ostream& out, std:
             app("external.dat", av.V);
             app("interaction.dat", av.I);
         }
+        if (last && dens_flags) {
+            // ref Sampler::print*DensityToFile (src/sampler.cpp:522-581)
+            const int nb = c.numberOfBins;
+            std::vector<int64_t> radial(nb), spatial((size_t)nb * nb), pair((size_t)nb * nb);
+            if (vmc_get_density(eng, radial.data(), spatial.data(), pair.data()) != VMC_OK)
+                return fail("densities");
+            if (c.computeOneBodyDensity) write_counts(c.path + "onebody.dat", radial, nb, 1);
+            if (c.computeOneBodyDensity2) write_counts(c.path + "onebody2.dat", spatial, nb, nb);
+            if (c.computeTwoBodyDensity) write_counts(c.path + "twobody.dat", pair, nb, nb);
+        }
         if (last) {
-            double E = av.E, seE = std::sqrt(av.varE);
-            if (c.doResampling) {
-                size_t n = 1;
-                while (n * 2 <= rec.size()) n *= 2;
-                std::vector<double> x(rec.begin(), rec.begin() + n);
-                double m, se;
-                if (blocking(x, m, se)) {
-                    E = m;
-                    seE = se;
+            // with resampling: blocking over the instant series written to
+            // <n>_0.dat, <n>_0_kin.dat, <n>_0_ext.dat, <n>_0_int.dat, one value per
+            // line, the recorded walkers' chains one after another as the
+            // reference appends its ranks' files; the three part files are
+            // removed afterwards, the energy file is kept (ref
+            // src/sampler.cpp:318-430, 466-482, 624-632)
+            double mean[4] = {av.E, av.T, av.V, av.I};
+            double se[4] = {std::sqrt(av.varE), std::sqrt(av.varT), std::sqrt(av.varV),
+                            std::sqrt(av.varI)};
+            if (c.doResampling && rw > 0) {
+                std::mt19937 pick((uint32_t)(seed ^ 0x1a57a17u));
+                const std::string stem =
+                    c.path + std::to_string(pick() % 1000000u) + "_0";
+                const char* ext[4] = {".dat", "_kin.dat", "_ext.dat", "_int.dat"};
+                const size_t plane = (size_t)rw * S;
+                for (int q = 0; q < 4; q++) {
+                    const std::string name = stem + ext[q];
+                    {
+                        std::ofstream f(name);
+                        for (size_t i = 0; i < plane; i++) f << double(rec[q * plane + i]) << "\n";
+                    }
+                    std::vector<double> x;
+                    {
+                        std::ifstream f(name);
+                        std::string line;
+                        while (std::getline(f, line)) x.push_back(std::strtod(line.c_str(), nullptr));
+                    }
+                    size_t n = 1;
+                    while (n * 2 <= x.size()) n *= 2;
+                    x.resize(n);
+                    double m, e;
+                    if (blocking(x, m, e)) {
+                        mean[q] = m;
+                        se[q] = e;
+                    }
+                    if (q > 0) std::remove(name.c_str());
                 }
             }
             out << std::endl << std::fixed << std::setprecision(10);
             out << "               Final results" << std::endl;
             out << "==============================================" << std::endl;
-            out << " Total energy            : " << E << " (with STD = " << seE << ")" << std::endl;
-            out << " Kinetic energy          : " << av.T << " (with STD = " << std::sqrt(av.varT) << ")" << std::endl;
-            out << " External energy         : " << av.V << " (with STD = " << std::sqrt(av.varV) << ")" << std::endl;
-            out << " Interaction energy      : " << av.I << " (with STD = " << std::sqrt(av.varI) << ")" << std::endl;
+            out << " Total energy            : " << mean[0] << " (with STD = " << se[0] << ")" << std::endl;
+            out << " Kinetic energy          : " << mean[1] << " (with STD = " << se[1] << ")" << std::endl;
+            out << " External energy         : " << mean[2] << " (with STD = " << se[2] << ")" << std::endl;
+            out << " Interaction energy      : " << mean[3] << " (with STD = " << se[3] << ")" << std::endl;
             out << " Acceptence Ratio        : " << av.acc << std::endl << std::endl;
             const auto end = std::chrono::system_clock::now();
             const std::chrono::duration<double> el = end - start;

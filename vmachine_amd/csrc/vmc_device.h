@@ -84,8 +84,20 @@ struct WalkParams {
     double* pos;        // [W][N*2] FP64, read at launch start, written at the end
     double* qf;         // [W][N*2] stale quantum force (importance compat), may be null
     double* acc;        // [W][kNumWalkerAcc] per-walker accumulators (overwritten)
-    float* rec;         // [rec_walkers][steps] local energies, may be null
+    float* rec;         // [rec_parts][rec_walkers][steps] instant values, may be null:
+                        // E_L, then (rec_parts == 4) kinetic, external, interaction
     int rec_walkers;
+    int rec_parts;      // 1 or 4
+    // electron densities counted after every sampled step (ref
+    // src/sampler.cpp:639-692): bit 0 radial one-body, bit 1 spatial
+    // one-body, bit 2 radial two-body; 0 = off
+    int dens_flags;
+    int dens_bins;
+    double dens_step;       // maxRadius / bins
+    double dens_max_radius;
+    unsigned long long* dens_radial;    // [bins]
+    unsigned long long* dens_spatial;   // [bins][bins], row = y cell, column = x cell
+    unsigned long long* dens_pair;      // [bins][bins], row = bin of the lower-index particle
     int init_qf;        // 1: set the stored drift to the drift at the loaded positions
 };
 

@@ -156,6 +156,10 @@ struct vmc_engine {
     double* d_out = nullptr;
     float* d_rec = nullptr;
     size_t rec_cap = 0;
+    // electron densities: one allocation, radial [bins] | spatial [bins^2] | pair [bins^2]
+    unsigned long long* d_dens = nullptr;
+    int dens_flags = 0, dens_bins = 0;
+    double dens_max_radius = 0;
 
     ~vmc_engine()
     {
@@ -164,6 +168,7 @@ struct vmc_engine {
         if (d_acc) (void)hipFree(d_acc);
         if (d_out) (void)hipFree(d_out);
         if (d_rec) (void)hipFree(d_rec);
+        if (d_dens) (void)hipFree(d_dens);
         for (double* b : {d_rbm_a, d_rbm_b, d_rbm_W, d_rbm_acc, d_rbm_out})
             if (b) (void)hipFree(b);
         for (void* b : {(void*)d_lp_theta, (void*)d_lp_bf16, (void*)d_lp_m, (void*)d_lp_v})
@@ -462,11 +467,23 @@ static vmc::WalkParams base_params(const vmc_engine* e)
     if (c.sampling == VMC_LEARNED) P.q2_upper = 0;   // the exact |psi'/psi|^2
     P.seed = e->seed;
     P.W = c.num_walkers;
+    P.rec_parts = 1;
+    if (e->dens_flags) {
+        const size_t b = (size_t)e->dens_bins;
+        P.dens_flags = e->dens_flags;
+        P.dens_bins = e->dens_bins;
+        P.dens_step = e->dens_max_radius / e->dens_bins;   // ref sampler.cpp:41
+        P.dens_max_radius = e->dens_max_radius;
+        P.dens_radial = e->d_dens;
+        P.dens_spatial = e->d_dens + b;
+        P.dens_pair = e->d_dens + b + b * b;
+    }
     return P;
 }
 
 static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batches,
-                    vmc_sums* sums, double* grad, double* grad_e, int rec_walkers, float* energies)
+                    vmc_sums* sums, double* grad, double* grad_e, int rec_walkers, float* energies,
+                    int rec_parts = 1)
 {
     if (!e) return fail(VMC_E_INVALID, "null engine");
     if (!e->params_set) return fail(VMC_E_STATE, "vmc_set_parameters has not been called");
@@ -479,7 +496,7 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
         return fail(VMC_E_INVALID, "record_walkers out of range");
     const int W = e->cfg.num_walkers;
     if (rec_walkers > 0) {
-        const size_t need = (size_t)rec_walkers * steps;
+        const size_t need = (size_t)rec_parts * rec_walkers * steps;
         if (need > e->rec_cap) {
             if (e->d_rec) (void)hipFree(e->d_rec);
             e->d_rec = nullptr;
@@ -500,6 +517,7 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
     P.qf = (e->cfg.sampling == VMC_IMPORTANCE) ? e->d_qf : nullptr;
     P.acc = e->d_acc;
     P.rec = rec_walkers > 0 ? e->d_rec : nullptr;
+    P.rec_parts = rec_parts;
     P.rec_walkers = rec_walkers;
     P.init_qf = e->init_qf ? 1 : 0;
     const int total = equil + steps;
@@ -523,7 +541,8 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
                                hipMemcpyDeviceToHost, e->stream));
     }
     if (rec_walkers > 0 && energies)
-        HIP_TRY(hipMemcpyAsync(energies, e->d_rec, sizeof(float) * (size_t)rec_walkers * steps,
+        HIP_TRY(hipMemcpyAsync(energies, e->d_rec,
+                               sizeof(float) * (size_t)rec_parts * rec_walkers * steps,
                                hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     float ms = 0;
@@ -589,6 +608,49 @@ int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration,
     if (record_walkers > 0 && !energies) return fail(VMC_E_INVALID, "null energies buffer");
     return run_impl(e, steps, equilibration, iteration, batches, sums, grad, grad_e,
                     record_walkers, energies);
+}
+
+int vmc_run_instant(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+                    vmc_sums* sums, double* grad, double* grad_e, int record_walkers, float* out)
+{
+    if (record_walkers < 1 || !out) return fail(VMC_E_INVALID, "record_walkers >= 1 and an output buffer");
+    return run_impl(e, steps, equilibration, iteration, batches, sums, grad, grad_e,
+                    record_walkers, out, 4);
+}
+
+int vmc_set_density(vmc_engine* e, int flags, int bins, double max_radius)
+{
+    if (!e) return fail(VMC_E_INVALID, "null engine");
+    if (flags & ~(VMC_DENSITY_RADIAL | VMC_DENSITY_SPATIAL | VMC_DENSITY_TWO_BODY))
+        return fail(VMC_E_INVALID, "unknown density flags");
+    if (flags && (bins < 1 || bins > VMC_DENSITY_MAX_BINS || !(max_radius > 0)))
+        return fail(VMC_E_INVALID, "numberOfBins in 1..4096 and maxRadius > 0");
+    if (e->d_dens) (void)hipFree(e->d_dens);
+    e->d_dens = nullptr;
+    e->dens_flags = 0;
+    if (!flags) return VMC_OK;
+    const size_t n = (size_t)bins + 2 * (size_t)bins * bins;
+    HIP_TRY(hipMalloc(&e->d_dens, n * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(e->d_dens, 0, n * sizeof(unsigned long long), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->dens_flags = flags;
+    e->dens_bins = bins;
+    e->dens_max_radius = max_radius;
+    return VMC_OK;
+}
+
+int vmc_get_density(vmc_engine* e, int64_t* radial, int64_t* spatial, int64_t* two_body)
+{
+    if (!e) return fail(VMC_E_INVALID, "null engine");
+    if (!e->dens_flags) return fail(VMC_E_STATE, "densities are off (vmc_set_density)");
+    const size_t b = (size_t)e->dens_bins;
+    int64_t* dst[3] = {radial, spatial, two_body};
+    const size_t off[3] = {0, b, b + b * b}, len[3] = {b, b * b, b * b};
+    for (int i = 0; i < 3; i++)
+        if (dst[i])
+            HIP_TRY(hipMemcpy(dst[i], e->d_dens + off[i], len[i] * sizeof(int64_t),
+                              hipMemcpyDeviceToHost));
+    return VMC_OK;
 }
 
 static int evaluate_impl(vmc_engine* e, const double* positions, int walkers, const int* moves,

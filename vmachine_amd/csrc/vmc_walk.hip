@@ -1305,9 +1305,58 @@ __device__ __forceinline__ WalkDyn walk_dyn(const WalkParams& P)
 template <int NP, int K>
 constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 2 * (64 / K) * NP; }
 
+// electron densities of the walker's current state (ref src/sampler.cpp:639-692;
+// r_i = |x_i|, ref Metropolis::calculateRadialVectorElement, metropolis.cpp:31-39):
+//   radial one-body: r < maxRadius, bin int(r/dr) < bins;
+//   spatial one-body: cell (int(y/2dr) + bins/2, int(x/2dr) + bins/2) with C
+//     truncation toward zero, both |int(.)| < bins/2 (the centre cell spans 4dr);
+//   radial two-body: for i < j, (int(r_i/dr) + 1, int(r_j/dr) + 1), both < bins.
+// Lanes count their own particles; the pair loop takes the other particles'
+// bins from their owner lanes.  Positions enter in FP64 without contraction, as
+// the reference computes them.
+template <typename R, int NP, int K, bool RB>
+__device__ __forceinline__ void density_sample(const Walker<R, NP, K, RB>& w, const WalkParams& P,
+                                               bool live)
+{
+#pragma clang fp contract(off)
+    const int nb = P.dens_bins, half = nb / 2;
+    const double dr = P.dens_step;
+    int b2[2] = {nb, nb};
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const double x = double(w.x[s][0]), y = double(w.x[s][1]);
+        const double r = ::sqrt(x * x + y * y);
+        const double q = r / dr;
+        const bool own = live && w.valid[s];
+        b2[s] = (q < double(nb - 1)) ? int(q) + 1 : nb;
+        if (own && (P.dens_flags & 1) && r < P.dens_max_radius && q < double(nb))
+            atomicAdd(P.dens_radial + int(q), 1ull);
+        if (own && (P.dens_flags & 2)) {
+            const double fx = x / (2.0 * dr), fy = y / (2.0 * dr);
+            // |int(f)| < half exactly when |f| < half (truncation toward zero)
+            if (fabs(fx) < double(half) && fabs(fy) < double(half))
+                atomicAdd(P.dens_spatial + (size_t)(int(fy) + half) * nb + (int(fx) + half), 1ull);
+        }
+    }
+    if (P.dens_flags & 4) {
+        for (int j = 0; j < P.N; j++) {
+            const int sj = j >= NP ? 1 : 0;
+            const int bj = __shfl(sj ? b2[1] : b2[0], w.gbase + (j - sj * NP), 64);
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const int q = s * NP + w.l;
+                if (live && w.valid[s] && q < j && b2[s] < nb && bj < nb)
+                    atomicAdd(P.dens_pair + (size_t)b2[s] * nb + bj, 1ull);
+            }
+        }
+    }
+}
+
 // the walk of the G = 64/K walkers of block `block` (one wave); lds_T and
-// lds_row as walk_static_lds_doubles, rbm_dyn the dynamic area
-template <typename R, int NP, int K, bool RB, bool LP>
+// lds_row as walk_static_lds_doubles, rbm_dyn the dynamic area.  DN: the
+// instantiation counts electron densities when P.dens_flags asks for them
+// (kept out of the plain instantiations the walk runs without densities)
+template <typename R, int NP, int K, bool RB, bool LP, bool DN = false>
 __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D, int block,
                                            double* lds_T, double* lds_row, double* rbm_dyn)
 {
@@ -1463,8 +1512,19 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                     }
                 }
             }
-            if (P.rec && live && walker < P.rec_walkers && w.l == 0)
-                P.rec[(size_t)walker * P.steps + (t - P.equil)] = float(E);
+            if (P.rec && live && walker < P.rec_walkers && w.l == 0) {
+                const size_t at = (size_t)walker * P.steps + (t - P.equil);
+                P.rec[at] = float(E);
+                if (P.rec_parts == 4) {
+                    const size_t plane = (size_t)P.rec_walkers * P.steps;
+                    P.rec[plane + at] = float(w.tk);
+                    P.rec[2 * plane + at] = float(w.vext);
+                    P.rec[3 * plane + at] = float(w.vint);
+                }
+            }
+            if constexpr (DN) {
+                if (P.dens_flags) density_sample(w, P, live);
+            }
         }
     }
 
@@ -1536,13 +1596,13 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     }
 }
 
-template <typename R, int NP, int K, bool RB, bool LP>
+template <typename R, int NP, int K, bool RB, bool LP, bool DN>
 __global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
 {
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     extern __shared__ double rbm_dyn[];
-    walk_block<R, NP, K, RB, LP>(P, walk_dyn(P), blockIdx.x, lds, lds + (64 / K) * NP * NP,
-                                 rbm_dyn);
+    walk_block<R, NP, K, RB, LP, DN>(P, walk_dyn(P), blockIdx.x, lds, lds + (64 / K) * NP * NP,
+                                     rbm_dyn);
 }
 
 // ------------------------------------------------------------------------
@@ -1678,14 +1738,14 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
 // ------------------------------------------------------------------------
 using RT = std::conditional_t<VMC_WALK_PRECISION == 32, float, double>;
 
-template <int NP, int K, bool RB, bool LP>
+template <int NP, int K, bool RB, bool LP, bool DN>
 static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
 {
     constexpr int G = 64 / K;
     const int blocks = (P.W + G - 1) / G;
     const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT))
                           : (LP ? lp_lds_bytes(K, sizeof(RT)) : 0);
-    hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP>), dim3(blocks), dim3(64), lds, stream, P);
+    hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN>), dim3(blocks), dim3(64), lds, stream, P);
     return hipGetLastError();
 }
 
@@ -1724,13 +1784,16 @@ hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, bool lp, const WalkP
     if (lp) {
         if (rbm) return hipErrorInvalidValue;
 #define Y(a, b) \
-    if (np == a && K == b) return launch_walk_t<a, b, false, true>(P, stream);
+    if (np == a && K == b) return launch_walk_t<a, b, false, true, true>(P, stream);
         VMC_LP_SHAPES(Y)
 #undef Y
         return hipErrorInvalidValue;
     }
-#define X(a, b, c) \
-    if (np == a && K == b && rbm == c) return launch_walk_t<a, b, c, false>(P, stream);
+    // the plain shapes have an instantiation without the density code
+#define X(a, b, c)                                                                  \
+    if (np == a && K == b && rbm == c)                                              \
+        return (c || P.dens_flags) ? launch_walk_t<a, b, c, false, true>(P, stream) \
+                                   : launch_walk_t<a, b, c, false, c>(P, stream);
     VMC_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG, "_lib", "libvmc_hip.so")
 HEADER = os.path.join(ROOT, "include", "vmc.h")
 
 GAUSSIAN, SLATER, PADE, RBM_GAUSSIAN, RBM_PRODUCT = 1, 2, 3, 4, 5
+DENSITY_RADIAL, DENSITY_SPATIAL, DENSITY_TWO_BODY = 1, 2, 4
 BRUTE_FORCE, IMPORTANCE, IMPORTANCE_EXACT, LEARNED = 0, 1, 2, 3
 LP_NPARAM = 5636
 Q2_PADE_RATIO_UPPER, Q3_PADE_GAMMA_HALF = 1 << 2, 1 << 3
@@ -114,6 +115,11 @@ def lib():
         L.vmc_lp_get.argtypes = [ctypes.c_void_p, _FP, _I]
         L.vmc_lp_gradient.argtypes = [ctypes.c_void_p, _DP, _I, _IP, _DP, _DP, _DP]
         L.vmc_lp_train.argtypes = [ctypes.c_void_p, _I, _I, _D, _DP]
+        L.vmc_run_instant.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, ctypes.POINTER(VmcSums),
+                                      _DP, _DP, _I, _FP]
+        L.vmc_set_density.argtypes = [ctypes.c_void_p, _I, _I, _D]
+        _I64P = ctypes.POINTER(ctypes.c_int64)
+        L.vmc_get_density.argtypes = [ctypes.c_void_p, _I64P, _I64P, _I64P]
         L.vmc_batch_create.argtypes = [ctypes.POINTER(VmcBatchConfig),
                                        ctypes.POINTER(ctypes.c_void_p)]
         L.vmc_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -260,6 +266,35 @@ class Engine:
         n = self.E * self.maxpar
         return (sums.as_dict(), g[:n].reshape(self.E, self.maxpar),
                 ge[:n].reshape(self.E, self.maxpar), en)
+
+    def run_instant(self, steps, equilibration=0, iteration=0, batches=1, record_walkers=1):
+        """vmc_run_instant: (sums, g, ge, out [4][record_walkers][steps]) with
+        out = local energy, kinetic, external, interaction (FP32)."""
+        sums = VmcSums()
+        g = np.zeros(max(self.E * self.maxpar, 1))
+        ge = np.zeros_like(g)
+        out = np.zeros((4, record_walkers, steps), dtype=np.float32)
+        self._check(self.L.vmc_run_instant(self.h, steps, equilibration, iteration, batches,
+                                           ctypes.byref(sums), _dp(g), _dp(ge), record_walkers,
+                                           out.ctypes.data_as(_FP)))
+        n = self.E * self.maxpar
+        return (sums.as_dict(), g[:n].reshape(self.E, self.maxpar),
+                ge[:n].reshape(self.E, self.maxpar), out)
+
+    def set_density(self, flags, bins=1000, max_radius=5.0):
+        self._check(self.L.vmc_set_density(self.h, flags, bins, max_radius))
+        self._dens_bins = bins
+
+    def density(self):
+        """(radial [bins], spatial [bins, bins], two_body [bins, bins]) int64 counts."""
+        b = getattr(self, "_dens_bins", 1)
+        r = np.zeros(b, dtype=np.int64)
+        sp = np.zeros((b, b), dtype=np.int64)
+        tb = np.zeros((b, b), dtype=np.int64)
+        P = ctypes.POINTER(ctypes.c_int64)
+        self._check(self.L.vmc_get_density(self.h, r.ctypes.data_as(P), sp.ctypes.data_as(P),
+                                           tb.ctypes.data_as(P)))
+        return r, sp, tb
 
     def evaluate(self, positions, moves, new_values):
         pos = np.ascontiguousarray(positions, dtype=np.float64)
