@@ -1088,6 +1088,27 @@ inline size_t lp_lds_bytes(int K, size_t rsize)
     return rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLpParams;
 }
 
+// minimum over the K lanes of a group of 64-bit keys (same pairing as gsum)
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, true);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+template <int K>
+__device__ __forceinline__ uint64_t gmin64(uint64_t v)
+{
+    auto take = [&](uint64_t o) { v = o < v ? o : v; };
+    if constexpr (K >= 2) take(dpp64<kDppXor1>(v));
+    if constexpr (K >= 4) take(dpp64<kDppXor2>(v));
+    if constexpr (K >= 8) take(dpp64<kDppHalfMirror>(v));
+    if constexpr (K >= 16) take(dpp64<kDppMirror>(v));
+    if constexpr (K >= 32) take(__shfl_xor(v, 16, 64));
+    if constexpr (K >= 64) take(__shfl_xor(v, 32, 64));
+    return v;
+}
+
 template <typename R, int NP, int K, bool RB>
 __device__ __forceinline__ void lp_features(const Walker<R, NP, K, RB>& w, const WalkParams& P, int i,
                                             R xi, R yi, R (&f)[kLpIn])
@@ -1095,6 +1116,35 @@ __device__ __forceinline__ void lp_features(const Walker<R, NP, K, RB>& w, const
     const R s = R(::sqrt(P.omega));
     f[0] = s * xi;
     f[1] = s * yi;
+    if constexpr (std::is_same<R, float>::value) {
+        // FP32: each candidate as one key (squared distance bits, index): the
+        // nearest remaining electron is the group minimum of the keys (the
+        // bits of a non-negative float order as the value; ties go to the
+        // lower index), taken out of its owner lane after each round
+        uint64_t key[2];
+#pragma unroll
+        for (int sl = 0; sl < 2; sl++) {
+            const int q = sl * NP + w.l;
+            const float e0 = w.x[sl][0] - xi, e1 = w.x[sl][1] - yi;
+            const float d2 = e0 * e0 + e1 * e1;
+            key[sl] = (w.valid[sl] && q != i) ? ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)q
+                                              : ~0ull;
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const uint64_t best = gmin64<K>(key[0] < key[1] ? key[0] : key[1]);
+            const bool found = best != ~0ull;
+            const int j = found ? (int)(uint32_t)best : 0;
+            const int js = j >= NP ? 1 : 0, jl = j - js * NP;
+            const R xj0 = __shfl(vsel(js != 0, w.x[1][0], w.x[0][0]), w.gbase + jl, 64);
+            const R xj1 = __shfl(vsel(js != 0, w.x[1][1], w.x[0][1]), w.gbase + jl, 64);
+            f[2 + 2 * m] = found ? s * (xj0 - xi) : R(0);
+            f[3 + 2 * m] = found ? s * (xj1 - yi) : R(0);
+            key[0] = key[0] == best ? ~0ull : key[0];
+            key[1] = key[1] == best ? ~0ull : key[1];
+        }
+        return;
+    }
     bool taken0 = false, taken1 = false;
     constexpr int kNone = 1 << 30;
     sfor<0, 8>([&](auto mi) {
@@ -1142,10 +1192,75 @@ __device__ __forceinline__ void lp_features(const Walker<R, NP, K, RB>& w, const
 __device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xFFFF0000u); }
 
+// FP32 walks: activations rounded to bfloat16 (nearest even) and the hidden
+// layers as bf16 dot products with FP32 accumulation (v_dot2c_f32_bf16), two
+// products per instruction with no weight conversion.  The proposal stays an
+// exact Metropolis-Hastings proposal: both passes evaluate the same function.
+typedef __bf16 lp_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float lp_float2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ lp_bf16x2 lp_pack(float a, float b)
+{
+    return __builtin_convertvector((lp_float2){a, b}, lp_bf16x2);
+}
+__device__ __forceinline__ lp_bf16x2 lp_word(uint32_t v) { return __builtin_bit_cast(lp_bf16x2, v); }
+__device__ __forceinline__ float lp_dot2(uint32_t w, uint32_t h, float c)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(lp_word(w), lp_word(h), c, false);
+}
+
+template <int K>
+__device__ __forceinline__ void lp_mlp_bf16(const LpLds<float, K>& L, int g, int l,
+                                            const float (&f)[kLpIn], float (&o)[kLpOut])
+{
+    constexpr int UPL = kLpHid / K;
+    constexpr int UNR = UPL <= 4 ? UPL : 1;
+    uint16_t* hb = reinterpret_cast<uint16_t*>(L.h) + g * kLpHid;   // bf16 activations
+    const uint32_t* W32 = reinterpret_cast<const uint32_t*>(L.W);
+    uint32_t fp[kLpIn / 2];
+#pragma unroll
+    for (int m = 0; m < kLpIn / 2; m++) fp[m] = __builtin_bit_cast(uint32_t, lp_pack(f[2 * m], f[2 * m + 1]));
+#pragma unroll UNR
+    for (int j = 0; j < UPL; j++) {
+        const int u = l + K * j;
+        float a = L.w(kLpB1 + u);
+        const uint32_t* row = W32 + (kLpW1 + u * kLpIn) / 2;
+#pragma unroll
+        for (int m = 0; m < kLpIn / 2; m++) a = lp_dot2(row[m], fp[m], a);
+        hb[u] = __builtin_bit_cast(uint16_t, (__bf16)lp_tanh(a));
+    }
+    __syncthreads();
+    float t[kLpOut] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll UNR
+    for (int j = 0; j < UPL; j++) {
+        const int u = l + K * j;
+        float a2 = L.w(kLpB2 + u);
+#pragma unroll
+        for (int c = 0; c < kLpHid; c += 8) {
+            const uint4 h = *reinterpret_cast<const uint4*>(hb + c);
+            const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLpW2 + u * kLpHid + c);
+            a2 = lp_dot2(v.x, h.x, a2);
+            a2 = lp_dot2(v.y, h.y, a2);
+            a2 = lp_dot2(v.z, h.z, a2);
+            a2 = lp_dot2(v.w, h.w, a2);
+        }
+        const float h2 = lp_tanh(a2);
+#pragma unroll
+        for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLpW3 + r * kLpHid + u) * h2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLpB3 + r);
+}
+
 template <typename R, int K>
 __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const R (&f)[kLpIn],
                                        R (&o)[kLpOut])
 {
+    if constexpr (std::is_same<R, float>::value) {
+        lp_mlp_bf16<K>(L, g, l, f, o);
+        return;
+    }
     constexpr int UPL = kLpHid / K;
     constexpr int UNR = UPL <= 4 ? UPL : 1;   // unroll only the few-units-per-lane shapes
     R* hb = L.h + g * kLpHid;
