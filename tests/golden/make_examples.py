@@ -30,6 +30,24 @@ FINAL = {
 }
 
 
+def parse_header(text):
+    """The information blocks printed before the iteration table: a list of
+    [title or key, value] in order (titles with value None)."""
+    out = []
+    for line in text.splitlines():
+        if line.startswith("Step  Energy"):
+            break
+        s = line.strip()
+        if not s or set(s) <= set("=_\\/|. `()-"):
+            continue
+        if s.isupper():
+            out.append([s, None])
+        elif ":" in s:
+            k, v = s.split(":", 1)
+            out.append([k.strip(), v.strip()])
+    return out
+
+
 def parse_log(text):
     rows, final = [], {}
     for line in text.splitlines():
@@ -78,8 +96,9 @@ def main():
         with open(os.path.join(OUT, name + ".in"), "w") as f:
             f.write("\n".join(keys) + "\n")
         with open(log, encoding="utf-8", errors="replace") as f:
-            rows, final = parse_log(f.read())
-        expected[name] = {"rows": rows, "final": final}
+            text = f.read()
+        rows, final = parse_log(text)
+        expected[name] = {"rows": rows, "final": final, "header": parse_header(text)}
         # output data files the example ships beside its log (e.g. the spatial
         # density onebody2.dat), verbatim
         for fn in sorted(os.listdir(d)):

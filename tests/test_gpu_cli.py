@@ -37,6 +37,47 @@ def _run(name, *extra, cwd=None):
     return np.array(rows), final, r.stdout
 
 
+def _header(text):
+    """The information blocks before the iteration table, as [key, value] pairs
+    (block titles with None), in the form tests/golden/make_examples.py keeps
+    the reference's."""
+    out = []
+    for line in text.splitlines():
+        if line.startswith("Step  Energy"):
+            break
+        s = line.strip()
+        if not s or set(s) <= set("=_\\/|. `()-"):
+            continue
+        if s.isupper():
+            out.append([s, None])
+        elif ":" in s:
+            k, v = s.split(":", 1)
+            out.append([k.strip(), v.strip()])
+    return out
+
+
+# lines that name this engine or the moment of the run rather than the input
+_OWN = ("Started computation at", "Random number generator", "Walkers and burn-in")
+
+
+# The RBM example's shipped log says "random uniform", but the reference's
+# source maps RBM-Gaussian x RBM-product ("RBM", ref src/parser.cpp:396-422)
+# to Xavier weights (ref src/InitialWeights/automatize.cpp:85-86); the log
+# predates that mapping, and the engine follows the source.
+_SOURCE_OVER_LOG = {("small_fermionic_quantumdot_rbm", "Initial parameters"): "Xavier"}
+
+
+def _check_header(name, text):
+    """The reference's terminal blocks (ref src/printinfo.cpp:45-122): the same
+    titles and keys in the same order and, for everything the input decides,
+    the same values."""
+    ref = [kv for kv in EXPECTED[name]["header"] if not kv[0].startswith(_OWN)]
+    ours = [kv for kv in _header(text) if not kv[0].startswith(_OWN)]
+    assert [k for k, _ in ours] == [k for k, _ in ref]
+    for (k, v), (_, rv) in zip(ours, ref):
+        assert v == _SOURCE_OVER_LOG.get((name, k), rv), (k, v, rv)
+
+
 def _compare(name, rows, final):
     ref = EXPECTED[name]
     R = np.array(ref["rows"])
@@ -54,7 +95,8 @@ def _compare(name, rows, final):
 
 
 def test_no_interaction_example_is_exact():
-    rows, final, _ = _run("no_interaction", "--seed", 11)
+    rows, final, out = _run("no_interaction", "--seed", 11)
+    _check_header("no_interaction", out)
     assert rows.shape == (99, 11)
     np.testing.assert_allclose(rows[:, 1], 2.0, rtol=0, atol=1e-12)
     assert final["energy"] == pytest.approx(2.0, abs=1e-10)
@@ -77,8 +119,9 @@ def _grid_r2(g, dr2):
 
 @pytest.mark.parametrize("name", ["small_fermionic_quantumdot_vmc", "electron_density"])
 def test_two_electron_dot_examples(name, tmp_path):
-    rows, final, _ = _run(name, "--seed", 12, cwd=tmp_path)
+    rows, final, out = _run(name, "--seed", 12, cwd=tmp_path)
     _compare(name, rows, final)
+    _check_header(name, out)
     if name == "electron_density":
         # the spatial one-body density of the last iteration (maxRadius 3, 100
         # bins) against the reference's own onebody2.dat: Eigen's integer
@@ -100,19 +143,22 @@ def test_two_electron_dot_examples(name, tmp_path):
 
 
 def test_twelve_electron_dot_example():
-    rows, final, _ = _run("large_fermionic_quantumdot_vmc", "--seed", 13)
+    rows, final, out = _run("large_fermionic_quantumdot_vmc", "--seed", 13)
     _compare("large_fermionic_quantumdot_vmc", rows, final)
+    _check_header("large_fermionic_quantumdot_vmc", out)
 
 
 def test_rbm_example():
     # RBM-Gaussian x RBM-product, N=2, H=2, Xavier weights, 500 ADAM iterations
-    rows, final, _ = _run("small_fermionic_quantumdot_rbm", "--seed", 17)
+    rows, final, out = _run("small_fermionic_quantumdot_rbm", "--seed", 17)
     _compare("small_fermionic_quantumdot_rbm", rows, final)
+    _check_header("small_fermionic_quantumdot_rbm", out)
 
 
 def test_resampling_example_blocks_the_last_iteration(tmp_path):
     rows, final, out = _run("resampling", "--seed", 14, cwd=tmp_path)
     _compare("resampling", rows, final)
+    _check_header("resampling", out)
     # the final standard errors come from blocking the recorded instant values:
     # correlated samples give an error above the naive sqrt(var/M) of the rows
     ref = EXPECTED["resampling"]["final"]

@@ -183,6 +183,25 @@ bool blocking(std::vector<double> x, double& mean, double& stderr_)
     return true;
 }
 
+// the preset initialWeights: automatize resolves to (ref automatize.cpp:83-101)
+static std::string resolved_weights_method(const Config& c)
+{
+    if (c.initialWeights != "automatize") return c.initialWeights;
+    const std::string label = trial_wave_function_label(c.elements);
+    if (label == "VMC" || label == "BVMC" || label == "PRBM") return "constant";
+    if (label == "RBM" || label == "RBMSJ" || label == "RBMPJ") return "xavier";
+    return "randomuniform";
+}
+
+std::string initial_weights_label(const Config& c)
+{
+    const std::string m = resolved_weights_method(c);   // labels of ref src/InitialWeights/*.h
+    if (m == "constant") return "constant";
+    if (m == "xavier") return "Xavier";
+    if (m == "fromfile") return "from-file";
+    return "random uniform";
+}
+
 std::vector<double> initial_weights(const Config& c, int rows, int cols, unsigned long long seed)
 {
     std::vector<double> P((size_t)rows * cols, 0.0);
@@ -328,31 +347,82 @@ int run_simulation(const Config& c, const RunOptions& o, std::ostream& out, std:
     opt.init(c.optimization, rows * cols, c.learningRate, c.optGamma, c.optMonotonic);
     const int B = opt.batches();
 
-    // ---- header (ref src/printinfo.cpp, src/system.cpp:83-91) ----
+    // ---- header: the reference's logo and information blocks with this run's
+    // values (ref src/printinfo.cpp:14-140, src/system.cpp:83-91) ----
+    out << "__      ____  __          _____ _    _ _____ _   _ ______ " << std::endl;
+    out << "\\ \\    / /  \\/  |   /\\   / ____| |  | |_   _| \\ | |  ____|" << std::endl;
+    out << " \\ \\  / /| \\  / |  /  \\ | |    | |__| | | | |  \\| | |__   " << std::endl;
+    out << "  \\ \\/ / | |\\/| | / /\\ \\| |    |  __  | | | | . ` |  __|  " << std::endl;
+    out << "   \\  /  | |  | |/ ____ \\ |____| |  | |_| |_| |\\  | |____ " << std::endl;
+    out << "    \\/   |_|  |_/_/    \\_\\_____|_|  |_|_____|_| \\_|______|" << std::endl;
+    out << "==========================================================" << std::endl << std::endl;
     std::time_t t0 = std::chrono::system_clock::to_time_t(start);
     out << "Started computation at " << std::ctime(&t0);
-    out << "Running on 1 MI355X (HIP) with " << W << " walkers, FP" << o.precision << " walk"
+    out << "Running on 1 MI355X GPU (HIP), " << W << " walkers, FP" << o.precision << " walk"
         << std::endl;
-    out << std::endl << "             SYSTEM INFORMATION" << std::endl;
-    out << "==============================================" << std::endl;
+    out << "Simulation is run from the directory: " << c.path << std::endl;
+    int nparams = 0;
+    for (int e = 0; e < rows; e++) nparams += vmc_num_parameters(&ec, e);
+    auto block = [&](const char* title) {
+        out << std::endl << std::endl << title << std::endl
+            << "==============================================" << std::endl;
+    };
+    out << std::fixed << std::boolalpha << std::setprecision(6);
+    block("             SYSTEM INFORMATION");
     out << "Number of particles:      " << c.numParticles << std::endl;
     out << "Number of dimensions:     " << c.numDimensions << std::endl;
-    out << "Interaction style:        " << (ec.interaction ? "Coulomb" : "No interaction") << std::endl;
+    out << "Interaction style:        " << (ec.interaction ? "Coulomb" : "no interaction") << std::endl;
     out << "Hamiltonian:              harmonic oscillator" << std::endl;
-    out << "Oscillator frequency:     " << std::fixed << std::setprecision(6) << c.omega << std::endl;
-    out << std::endl << "           WAVE FUNCTION INFORMATION" << std::endl;
-    out << "==============================================" << std::endl;
+    out << "Initial state:            random normal" << std::endl;
+    out << "Oscillator frequency:     " << c.omega << std::endl;
+    block("           WAVE FUNCTION INFORMATION");
     for (int e = 0; e < rows; e++)
         out << "Element " << e << ":                " << element_name(ec.kinds[e]) << std::endl;
-    out << "Trial wave function:      " << trial_wave_function_label(c.elements) << std::endl;
+    out << "Basis:                    " << (c.basis == "hermite" ? "Hermite" : "no basis") << std::endl;
+    out << "Initial parameters:       " << initial_weights_label(c) << std::endl;
+    out << "Number of parameters:     " << nparams << std::endl;
+    out << "Number of hidden nodes:   " << c.numHiddenUnits << std::endl;
+    out << "Print parameters to file: " << c.printParametersToFile << std::endl;
+    block("            SIMULATION INFORMATION");
+    out << "Max number of iterations: " << c.numIterations << std::endl;
+    out << "Number of MC cycles:      " << c.totalStepsWOEqui << std::endl;
+    out << "Number of burn-in cycles: " << c.equilibrationSteps << std::endl;
+    out << "Learning rate:            " << c.learningRate << std::endl;
+    out << "Step length:              " << c.stepLength << std::endl;
+    out << "Equilibration fraction:   " << c.equilibrationFraction << std::endl;
     out << "Sampling:                 "
-        << (ec.sampling == VMC_BRUTE_FORCE ? "brute force"
-                                           : (o.exact ? "Langevin, exact Metropolis-Hastings"
-                                                      : "Metropolis-Hastings (reference ratio)"))
+        << (ec.sampling == VMC_BRUTE_FORCE
+                ? "brute force Metropolis"
+                : (o.exact ? "Metropolis-Hastings (Langevin, exact ratio)" : "Metropolis-Hastings"))
         << std::endl;
-    out << "Walker burn-in:           " << o.burn_in_sweeps << " sweeps (" << o.burn_in_sweeps * dof
-        << " steps per walker)" << std::endl;
-    out << "Random seed:              " << seed << std::endl << std::endl;
+    out << "Optimization:             "
+        << (c.optimization == "adam" ? "ADAM"
+                                     : (c.optimization == "sgd" ? "stochastic gradient descent"
+                                                                : "gradient descent"))
+        << std::endl;
+    out << "Random number generator:  Philox4x32-10 (seed " << seed << ")" << std::endl;
+    out << "Walkers and burn-in:      " << W << " walkers, " << o.burn_in_sweeps << " sweeps ("
+        << o.burn_in_sweeps * dof << " steps per walker)" << std::endl;
+    block("               PARTICLE DENSITY");
+    out << "Compute radial one-body density:  " << c.computeOneBodyDensity << std::endl;
+    out << "Compute spatial one-body density: " << c.computeOneBodyDensity2 << std::endl;
+    out << "Compute radial two-body density:  " << c.computeTwoBodyDensity << std::endl;
+    out << "Max radius of electron density:   " << c.maxRadius << std::endl;
+    out << "Number of bins in each direction: " << c.numberOfBins << std::endl;
+    block("                  RESAMPLING");
+    out << "Do resampling:            " << c.doResampling << std::endl;
+    out << "Print energies to file:   " << c.printEnergyToFile << std::endl;
+    block("                  CONVERGENCE");
+    out << "Check convergence:        " << c.checkConvergence << std::endl;
+    out << "Tolerance:                " << c.tolerance << std::endl;
+    out << "Number of energies:       " << c.numberOfEnergies << std::endl;
+    block("                 ADAPTIVE STEPS");
+    out << "Apply adaptive steps:       " << c.applyAdaptiveSteps << std::endl;
+    out << "Range of adaptive steps:    " << c.rangeOfAdaptiveSteps << std::endl;
+    out << "Additional steps:           " << c.additionalSteps << std::endl;
+    out << "Additional steps last iter: " << c.additionalStepsLastIter << std::endl;
+    out << std::endl << std::endl;
+    out << std::noboolalpha;
     out << "Step  Energy  Energy_STD  Kinetic  Kinetic_STD  External  External_STD  "
            "Interaction  Interaction_STD  Acceptence  CPU_time" << std::endl;
 

@@ -46,6 +46,8 @@ struct Optimizer {
 // initialWeights (ref src/InitialWeights/automatize.cpp:55-105, constant.cpp,
 // xavier.cpp, randomuniform.cpp, fromfile.cpp).
 std::vector<double> initial_weights(const Config& c, int rows, int cols, unsigned long long seed);
+// the reference's label of the initial weights (the method automatize resolves to)
+std::string initial_weights_label(const Config& c);
 
 // Runs the simulation and writes the reference's terminal output to `out`.
 // Returns the process exit status (0, or 143 for configuration errors as the
