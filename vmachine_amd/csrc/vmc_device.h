@@ -38,6 +38,9 @@ struct WalkParams {
     int N;              // particles
     int n_up;           // particles 0..n_up-1 are spin up (reference: i < N/2)
     double omega;
+    double sqrt_omega;      // sqrt(omega), 1/sqrt(omega), sqrt(dt): set by the host
+    double inv_sqrt_omega;
+    double sqrt_dt;
     // elements present
     int has_gauss, has_slater, has_pade, coulomb;
     double alpha;       // Gaussian alpha

@@ -434,6 +434,9 @@ static vmc::WalkParams base_params(const vmc_engine* e)
     P.N = c.num_particles;
     P.n_up = c.num_particles / 2;
     P.omega = c.omega;
+    P.sqrt_omega = std::sqrt(c.omega);
+    P.inv_sqrt_omega = 1.0 / std::sqrt(c.omega);
+    P.sqrt_dt = std::sqrt(c.step_length);
     P.has_gauss = e->row_gauss >= 0;
     P.has_slater = has(&c, VMC_SLATER);
     P.has_pade = e->row_pade >= 0;
@@ -1122,6 +1125,9 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         P.N = y.num_particles;
         P.n_up = y.num_particles / 2;
         P.omega = y.omega;
+        P.sqrt_omega = std::sqrt(y.omega);
+        P.inv_sqrt_omega = 1.0 / std::sqrt(y.omega);
+        P.sqrt_dt = std::sqrt(y.step_length);
         for (int e = 0; e < c.num_elements; e++) {
             P.has_gauss |= c.kinds[e] == VMC_GAUSSIAN;
             P.has_slater |= c.kinds[e] == VMC_SLATER;

@@ -297,7 +297,7 @@ struct Walker {
     __device__ __forceinline__ bool hvalid(const WalkParams& P, int j) const { return l + K * j < P.H; }
     __device__ __forceinline__ R isp2(const WalkParams& P) const { return R(P.isig2_p); }
 
-    __device__ __forceinline__ R sw(const WalkParams& P) const { return R(::sqrt(P.omega)); }
+    __device__ __forceinline__ R sw(const WalkParams& P) const { return R(P.sqrt_omega); }
 
     // Slater log-derivatives of a particle at (px, py) from its column `col` of
     // the block inverse
@@ -691,7 +691,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
 {
     using WK = Walker<R, NP, K, RB>;
     if (pm) k = 2 * pm->p;
-    const R dt = R(P.dt), sqdt = R(::sqrt(P.dt));
+    const R dt = R(P.dt), sqdt = R(P.sqrt_dt);
     const R oa = R(P.omega * w.th_alpha);
     const R gam = R(w.th_gamma);
     const int p = k >> 1, d = k & 1;
@@ -1088,22 +1088,15 @@ inline size_t lp_lds_bytes(int K, size_t rsize)
     return rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLpParams;
 }
 
-// minimum over the K lanes of a group of 64-bit keys (same pairing as gsum)
-template <int CTRL>
-__device__ __forceinline__ uint64_t dpp64(uint64_t v)
-{
-    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, true);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
-}
+// minimum over the K lanes of a group of 32-bit keys (same pairing as gsum)
 template <int K>
-__device__ __forceinline__ uint64_t gmin64(uint64_t v)
+__device__ __forceinline__ uint32_t gmin32(uint32_t v)
 {
-    auto take = [&](uint64_t o) { v = o < v ? o : v; };
-    if constexpr (K >= 2) take(dpp64<kDppXor1>(v));
-    if constexpr (K >= 4) take(dpp64<kDppXor2>(v));
-    if constexpr (K >= 8) take(dpp64<kDppHalfMirror>(v));
-    if constexpr (K >= 16) take(dpp64<kDppMirror>(v));
+    auto take = [&](uint32_t o) { v = o < v ? o : v; };
+    if constexpr (K >= 2) take((uint32_t)dpp<kDppXor1>((int)v));
+    if constexpr (K >= 4) take((uint32_t)dpp<kDppXor2>((int)v));
+    if constexpr (K >= 8) take((uint32_t)dpp<kDppHalfMirror>((int)v));
+    if constexpr (K >= 16) take((uint32_t)dpp<kDppMirror>((int)v));
     if constexpr (K >= 32) take(__shfl_xor(v, 16, 64));
     if constexpr (K >= 64) take(__shfl_xor(v, 32, 64));
     return v;
@@ -1113,35 +1106,37 @@ template <typename R, int NP, int K, bool RB>
 __device__ __forceinline__ void lp_features(const Walker<R, NP, K, RB>& w, const WalkParams& P, int i,
                                             R xi, R yi, R (&f)[kLpIn])
 {
-    const R s = R(::sqrt(P.omega));
+    const R s = R(P.sqrt_omega);
     f[0] = s * xi;
     f[1] = s * yi;
     if constexpr (std::is_same<R, float>::value) {
-        // FP32: each candidate as one key (squared distance bits, index): the
-        // nearest remaining electron is the group minimum of the keys (the
-        // bits of a non-negative float order as the value; ties go to the
-        // lower index), taken out of its owner lane after each round
-        uint64_t key[2];
+        // FP32: each candidate as one 32-bit key, the squared distance's bits
+        // with the low 6 replaced by the electron's index (the bits of a
+        // non-negative float order as the value): the nearest remaining
+        // electron is the group minimum of the keys, taken out of its owner
+        // lane after each round.  Distances within 2^-17 of each other order
+        // by index; the selection is a fixed function of the configuration,
+        // the same in both passes of the proposal.
+        uint32_t key[2];
 #pragma unroll
         for (int sl = 0; sl < 2; sl++) {
             const int q = sl * NP + w.l;
             const float e0 = w.x[sl][0] - xi, e1 = w.x[sl][1] - yi;
             const float d2 = e0 * e0 + e1 * e1;
-            key[sl] = (w.valid[sl] && q != i) ? ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)q
-                                              : ~0ull;
+            key[sl] = (w.valid[sl] && q != i) ? ((__float_as_uint(d2) & ~63u) | (uint32_t)q) : ~0u;
         }
 #pragma unroll
         for (int m = 0; m < 8; m++) {
-            const uint64_t best = gmin64<K>(key[0] < key[1] ? key[0] : key[1]);
-            const bool found = best != ~0ull;
-            const int j = found ? (int)(uint32_t)best : 0;
+            const uint32_t best = gmin32<K>(key[0] < key[1] ? key[0] : key[1]);
+            const bool found = best != ~0u;
+            const int j = found ? (int)(best & 63u) : 0;
             const int js = j >= NP ? 1 : 0, jl = j - js * NP;
             const R xj0 = __shfl(vsel(js != 0, w.x[1][0], w.x[0][0]), w.gbase + jl, 64);
             const R xj1 = __shfl(vsel(js != 0, w.x[1][1], w.x[0][1]), w.gbase + jl, 64);
             f[2 + 2 * m] = found ? s * (xj0 - xi) : R(0);
             f[3 + 2 * m] = found ? s * (xj1 - yi) : R(0);
-            key[0] = key[0] == best ? ~0ull : key[0];
-            key[1] = key[1] == best ? ~0ull : key[1];
+            key[0] = key[0] == best ? ~0u : key[0];
+            key[1] = key[1] == best ? ~0u : key[1];
         }
         return;
     }
@@ -1328,16 +1323,18 @@ __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const
 
 // move mean and scale: mu = l_mu tanh(o/l_mu) / sqrt(omega), scale =
 // sigma0 exp(ls tanh(o/ls)) / sqrt(omega)
+// (lt = ls tanh(o/ls), the log of the scale relative to sigma0/sqrt(omega))
 template <typename R>
 __device__ __forceinline__ void lp_head(const WalkParams& P, const R (&o)[kLpOut], R (&mu)[2],
-                                        R (&sg)[2])
+                                        R (&sg)[2], R (&lt)[2])
 {
-    const R ell = R(1.0 / ::sqrt(P.omega));
+    const R ell = R(P.inv_sqrt_omega);
     const R mm = R(P.lp_mu_max), ls = R(P.lp_ls_max);
 #pragma unroll
     for (int d = 0; d < 2; d++) {
         mu[d] = ell * mm * lp_tanh(o[d] / mm);
-        sg[d] = R(P.lp_sigma0) * ell * M<R>::exp(ls * lp_tanh(o[2 + d] / ls));
+        lt[d] = ls * lp_tanh(o[2 + d] / ls);
+        sg[d] = R(P.lp_sigma0) * ell * M<R>::exp(lt[d]);
     }
 }
 
@@ -1351,15 +1348,26 @@ __device__ __forceinline__ ParticleMove<R> lp_propose(const Walker<R, NP, K, RB>
     const int slot = i >= NP ? 1 : 0, li = i - slot * NP;
     const R xi = __shfl(vsel(slot != 0, w.x[1][0], w.x[0][0]), w.gbase + li, 64);
     const R yi = __shfl(vsel(slot != 0, w.x[1][1], w.x[0][1]), w.gbase + li, 64);
-    R f[kLpIn], o[kLpOut], mu[2], sg[2];
+    R f[kLpIn], o[kLpOut], mu[2], sg[2], lt[2];
     lp_features(w, P, i, xi, yi, f);
     lp_mlp(L, w.g, w.l, f, o);
-    lp_head(P, o, mu, sg);
+    lp_head(P, o, mu, sg, lt);
     const R xn = xi + mu[0] + sg[0] * z0, yn = yi + mu[1] + sg[1] * z1;
+    if constexpr (std::is_same<R, float>::value) {
+        // log sigma = log(sigma0/sqrt(omega)) + lt: the constant cancels in the
+        // ratio, which then needs no logarithm
+        const R lqf = -R(0.5) * (z0 * z0 + z1 * z1) - (lt[0] + lt[1]);
+        lp_features(w, P, i, xn, yn, f);
+        lp_mlp(L, w.g, w.l, f, o);
+        lp_head(P, o, mu, sg, lt);
+        const R r0 = (xi - xn - mu[0]) * M<R>::rcp(sg[0]), r1 = (yi - yn - mu[1]) * M<R>::rcp(sg[1]);
+        const R lqr = -R(0.5) * (r0 * r0 + r1 * r1) - (lt[0] + lt[1]);
+        return ParticleMove<R>{i, xn, yn, lqr - lqf};
+    }
     const R lqf = -(R(0.5) * z0 * z0 + M<R>::log(sg[0])) - (R(0.5) * z1 * z1 + M<R>::log(sg[1]));
     lp_features(w, P, i, xn, yn, f);
     lp_mlp(L, w.g, w.l, f, o);
-    lp_head(P, o, mu, sg);
+    lp_head(P, o, mu, sg, lt);
     const R r0 = xi - xn - mu[0], r1 = yi - yn - mu[1];
     const R lqr = -(r0 * r0 / (R(2) * sg[0] * sg[0]) + M<R>::log(sg[0])) -
                   (r1 * r1 / (R(2) * sg[1] * sg[1]) + M<R>::log(sg[1]));
