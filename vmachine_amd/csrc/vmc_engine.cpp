@@ -21,6 +21,7 @@ hipError_t launch_walk(int precision, const KernelShape& sh, const WalkParams& P
 hipError_t launch_eval(int precision, const KernelShape& sh, const WalkParams& P,
                        const double* pos, const int* moves, const double* newv, const EvalOut& O,
                        hipStream_t stream);
+// out: [1 + kRowChunks][width] (the sums, then partial rows)
 hipError_t launch_reduce_rows(const double* rows, int nrows, int width, double* out,
                               hipStream_t stream);
 hipError_t launch_lp_draws(const LpTrainParams& T, uint64_t iteration, int* electrons,
@@ -42,6 +43,7 @@ hipError_t launch_batch_tail(const BatchTable* T, int nsys, BatchOpt* opt, doubl
 int batch_shape_id(int np, int K);
 size_t batch_lds_bytes(int np, int K);
 int batch_resident_per_cu(int precision, size_t lds);
+// out: [1 + ceil(W/256)][kNumWalkerAcc] (the sums, then block partials)
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
                                  hipStream_t stream);
@@ -67,6 +69,7 @@ int fail(int code, const std::string& msg)
     } while (0)
 
 const int kNP[] = {1, 3, 6, 10, 15, 21, 28};
+constexpr int kRowChunks = 64;   // partial rows of launch_reduce_rows
 const int kLanes[] = {1, 4, 8, 16, 16, 32, 32};   // lanes per walker for each kNP entry
 
 // RBM instantiations: (np, K) pairs compiled with the RBM elements
@@ -306,7 +309,8 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
         (err = hipMalloc(&e->d_pos, W * e->nd * sizeof(double))) != hipSuccess ||
         (err = hipMalloc(&e->d_qf, W * e->nd * sizeof(double))) != hipSuccess ||
         (err = hipMalloc(&e->d_acc, W * vmc::kNumWalkerAcc * sizeof(double))) != hipSuccess ||
-        (err = hipMalloc(&e->d_out, vmc::kNumWalkerAcc * sizeof(double))) != hipSuccess)
+        (err = hipMalloc(&e->d_out, (size_t)(1 + (c.num_walkers + 255) / 256) *
+                                        vmc::kNumWalkerAcc * sizeof(double))) != hipSuccess)
         return cleanup(fail(VMC_E_DEVICE, std::string("allocation: ") + hipGetErrorString(err)));
     if (rbm) {
         const int H = has(&c, VMC_RBM_PRODUCT) ? c.num_hidden : 0, nd = e->nd;
@@ -317,7 +321,8 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
             (err = hipMalloc(&e->d_rbm_W, ((size_t)nd * H + 1) * sizeof(double))) != hipSuccess ||
             (err = hipMalloc(&e->d_rbm_acc, (size_t)e->rbm_blocks * e->rbm_width * sizeof(double))) !=
                 hipSuccess ||
-            (err = hipMalloc(&e->d_rbm_out, (size_t)e->rbm_width * sizeof(double))) != hipSuccess)
+            (err = hipMalloc(&e->d_rbm_out, (size_t)(1 + kRowChunks) * e->rbm_width * sizeof(double))) !=
+                hipSuccess)
             return cleanup(fail(VMC_E_DEVICE, std::string("allocation: ") + hipGetErrorString(err)));
     }
     *out = e;
@@ -809,7 +814,7 @@ static int lp_buffers(vmc_engine* e, int B)
     HIP_TRY(hipMalloc(&e->d_lp_o1, sizeof(double) * w9));
     HIP_TRY(hipMalloc(&e->d_lp_stats, sizeof(double) * 5 * B));
     HIP_TRY(hipMalloc(&e->d_lp_rows, sizeof(double) * (size_t)vmc::lp_wgrad_rows(B) * vmc::kLpParams));
-    HIP_TRY(hipMalloc(&e->d_lp_grad, sizeof(double) * vmc::kLpParams));
+    HIP_TRY(hipMalloc(&e->d_lp_grad, sizeof(double) * (1 + kRowChunks) * vmc::kLpParams));
     HIP_TRY(hipMalloc(&e->d_lp_act, sizeof(float) * (size_t)B * vmc::lp_act_floats()));
     HIP_TRY(hipMalloc(&e->d_lp_delta, sizeof(float) * (size_t)B * vmc::lp_delta_floats()));
     e->lp_B = B;

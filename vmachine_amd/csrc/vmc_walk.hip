@@ -1963,21 +1963,49 @@ hipError_t VMC_SUFFIX(launch_draws)(uint64_t seed, uint32_t walker, uint64_t cou
 // ------------------------------------------------------------------------
 // deterministic reduction of the per-walker accumulators: one block, fixed order
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void reduce_acc_kernel(const double* acc, int W, double* out)
+// column sums of the per-walker accumulators in two deterministic stages:
+// block b sums walkers [256 b, 256 b + 256) with a fixed tree (coalesced loads
+// of the block's rows into LDS), then one block sums the block partials in
+// order with the same tree
+__global__ __launch_bounds__(256) void reduce_acc_partial_kernel(const double* acc, int W,
+                                                                 double* part)
 {
-    __shared__ double part[256];
-    for (int a = 0; a < kNumWalkerAcc; a++) {
-        double s = 0;
-        for (int i = threadIdx.x; i < W; i += 256) s += acc[(size_t)i * kNumWalkerAcc + a];
-        part[threadIdx.x] = s;
-        __syncthreads();
-        for (int m = 128; m > 0; m >>= 1) {
-            if ((int)threadIdx.x < m) part[threadIdx.x] += part[threadIdx.x + m];
-            __syncthreads();
+    __shared__ double tile[256 * kNumWalkerAcc];
+    const int w0 = blockIdx.x * 256;
+    const int n = (W - w0 < 256 ? W - w0 : 256) * kNumWalkerAcc;
+    const double* src = acc + (size_t)w0 * kNumWalkerAcc;
+    for (int i = threadIdx.x; i < 256 * kNumWalkerAcc; i += 256) tile[i] = i < n ? src[i] : 0.0;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m) {
+#pragma unroll
+            for (int a = 0; a < kNumWalkerAcc; a++)
+                tile[threadIdx.x * kNumWalkerAcc + a] += tile[(threadIdx.x + m) * kNumWalkerAcc + a];
         }
-        if (threadIdx.x == 0) out[a] = part[0];
         __syncthreads();
     }
+    if ((int)threadIdx.x < kNumWalkerAcc)
+        part[(size_t)blockIdx.x * kNumWalkerAcc + threadIdx.x] = tile[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void reduce_acc_kernel(const double* part, int nb, double* out)
+{
+    __shared__ double tile[256 * kNumWalkerAcc];
+    for (int a = 0; a < kNumWalkerAcc; a++) {
+        double s = 0;
+        for (int i = threadIdx.x; i < nb; i += 256) s += part[(size_t)i * kNumWalkerAcc + a];
+        tile[threadIdx.x * kNumWalkerAcc + a] = s;
+    }
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m) {
+#pragma unroll
+            for (int a = 0; a < kNumWalkerAcc; a++)
+                tile[threadIdx.x * kNumWalkerAcc + a] += tile[(threadIdx.x + m) * kNumWalkerAcc + a];
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < kNumWalkerAcc) out[threadIdx.x] = tile[threadIdx.x];
 }
 
 // initial positions x ~ N(0, (1/omega)^2) (ref InitialStates/randomnormal.cpp:17)
@@ -2035,27 +2063,52 @@ hipError_t launch_eval(int precision, const KernelShape& sh, const WalkParams& P
 }
 
 // column sums of a row-major [rows][width] matrix, rows added in order
-__global__ __launch_bounds__(256) void reduce_rows_kernel(const double* rows, int nrows, int width,
+// column sums of a [nrows][width] array in two deterministic stages: chunk r
+// of the rows into partial row r, then the partial rows in order
+constexpr int kRowChunks = 64;
+
+__global__ __launch_bounds__(256) void reduce_rows_partial_kernel(const double* rows, int nrows,
+                                                                  int width, int chunk,
+                                                                  double* part)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= width) return;
+    const int r0 = blockIdx.y * chunk, r1 = r0 + chunk < nrows ? r0 + chunk : nrows;
+    double s = 0;
+    for (int r = r0; r < r1; r++) s += rows[(size_t)r * width + c];
+    part[(size_t)blockIdx.y * width + c] = s;
+}
+
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const double* part, int nparts, int width,
                                                           double* out)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= width) return;
     double s = 0;
-    for (int r = 0; r < nrows; r++) s += rows[(size_t)r * width + c];
+    for (int r = 0; r < nparts; r++) s += part[(size_t)r * width + c];
     out[c] = s;
 }
 
+// out: [1 + kRowChunks][width]: the sums, then the partial rows
 hipError_t launch_reduce_rows(const double* rows, int nrows, int width, double* out,
                               hipStream_t stream)
 {
-    hipLaunchKernelGGL(reduce_rows_kernel, dim3((width + 255) / 256), dim3(256), 0, stream, rows,
-                       nrows, width, out);
+    const int nparts = nrows / 32 < 1 ? 1 : (nrows / 32 < kRowChunks ? nrows / 32 : kRowChunks);
+    const int chunk = (nrows + nparts - 1) / nparts;
+    double* part = out + width;
+    const int cb = (width + 255) / 256;
+    hipLaunchKernelGGL(reduce_rows_partial_kernel, dim3(cb, nparts), dim3(256), 0, stream, rows,
+                       nrows, width, chunk, part);
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(cb), dim3(256), 0, stream, part, nparts, width, out);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream)
 {
-    hipLaunchKernelGGL(reduce_acc_kernel, dim3(1), dim3(256), 0, stream, acc, W, out);
+    const int nb = (W + 255) / 256;
+    double* part = out + kNumWalkerAcc;
+    hipLaunchKernelGGL(reduce_acc_partial_kernel, dim3(nb), dim3(256), 0, stream, acc, W, part);
+    hipLaunchKernelGGL(reduce_acc_kernel, dim3(1), dim3(256), 0, stream, part, nb, out);
     return hipGetLastError();
 }
 
