@@ -189,7 +189,7 @@ int batch_shape_id(int np, int K)
 // LDS of one batch workgroup: the largest Slater staging over the shapes used
 size_t batch_lds_bytes(int np, int K)
 {
-    return sizeof(double) * ((size_t)(64 / K) * np * np + 2 * (size_t)(64 / K) * np);
+    return sizeof(double) * ((size_t)(64 / K) * np * np + 5 * (size_t)(64 / K) * np);
 }
 
 // occupancy of the batch kernel: resident one-wave workgroups per CU with

@@ -192,6 +192,13 @@ __device__ __forceinline__ int dpp(int v)
 {
     return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp(uint64_t v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, true);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
 
 // sum over the K lanes of a group, every lane receives it.  Each step adds the
 // value of a partner lane, and partners pair up symmetrically, so all lanes
@@ -224,17 +231,16 @@ __device__ __forceinline__ T gprod(T v)
 }
 
 template <int K>
-__device__ __forceinline__ void gargmax(double& v, int& idx)
+__device__ __forceinline__ uint32_t gmax32(uint32_t v)
 {
-#pragma unroll
-    for (int m = K / 2; m >= 1; m >>= 1) {
-        const double ov = __shfl_xor(v, m, 64);
-        const int oi = __shfl_xor(idx, m, 64);
-        if (ov > v || (ov == v && oi < idx)) {
-            v = ov;
-            idx = oi;
-        }
-    }
+    auto take = [&](uint32_t o) { v = o > v ? o : v; };
+    if constexpr (K >= 2) take((uint32_t)dpp<kDppXor1>((int)v));
+    if constexpr (K >= 4) take((uint32_t)dpp<kDppXor2>((int)v));
+    if constexpr (K >= 8) take((uint32_t)dpp<kDppHalfMirror>((int)v));
+    if constexpr (K >= 16) take((uint32_t)dpp<kDppMirror>((int)v));
+    if constexpr (K >= 32) take(__shfl_xor(v, 16, 64));
+    if constexpr (K >= 64) take(__shfl_xor(v, 32, 64));
+    return v;
 }
 
 template <typename T>
@@ -333,69 +339,92 @@ struct Walker {
         sec = s * s * ss;
     }
 
-    // FP64 Gauss-Jordan inverse of spin block b; leaves column l in `out`.
-    __device__ __forceinline__ void invert_block(const WalkParams& P, int b, double* lds_T, double* lds_row,
-                                 R (&out)[NP])
+    // FP64 Gauss-Jordan inverses of the two spin blocks with partial
+    // pivoting, the pivot chains of the two blocks interleaved; leaves column l
+    // of the inverse of block 0 in out0 and of block 1 in out1.  Lane l holds
+    // row l of each block (orbital values of particle b NP + l).  lds_row:
+    // [2 buffers][2 blocks][G][NP] pivot rows, then [2][G][NP] pivot lanes as int.
+    __device__ __forceinline__ void invert_blocks(const WalkParams& P, double* lds_T,
+                                                  double* lds_row, R (&out0)[NP], R (&out1)[NP])
     {
-        const double s = ::sqrt(P.omega);
-        double a[NP];
-        {
+        const double s = P.sqrt_omega;
+        double a[2][NP];
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
             const double t0 = s * double(x[b][0]), t1 = s * double(x[b][1]);
             double hx[S + 1], hy[S + 1];
             hermite<double, S>(t0, hx);
             hermite<double, S>(t1, hy);
 #pragma unroll
-            for (int k = 0; k < NP; k++) a[k] = hx[orb_nx(k)] * hy[orb_ny(k)];
+            for (int k = 0; k < NP; k++) a[b][k] = hx[orb_nx(k)] * hy[orb_ny(k)];
         }
-        bool used = (l >= NP);
-        int mycol = 0;
-        int rowof[NP];
+        int* piv = reinterpret_cast<int*>(lds_row + 4 * G * NP);   // [2][G][NP]
+        bool used[2] = {l >= NP, l >= NP};
+        int mycol[2] = {0, 0};
         sfor<0, NP>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
-            double v = used ? -1.0 : fabs(a[c]);
-            int idx = l;
-            gargmax<K>(v, idx);
-            rowof[c] = idx;
-            double* row = lds_row + ((c & 1) * G + g) * NP;
-            if (l == idx) {
+            // the pivot: the group maximum of one 32-bit key per lane, the high
+            // word of |a| (sign, exponent and 20 mantissa bits, which order as
+            // the value) with the low 5 bits replaced by the lane, found with
+            // DPP steps; magnitudes within 2^-15 of each other count as equal
+            int idx[2];
 #pragma unroll
-                for (int j = 0; j < NP; j++) row[j] = a[j];
+            for (int b = 0; b < 2; b++) {
+                const uint32_t hiw = (uint32_t)(__builtin_bit_cast(uint64_t, fabs(a[b][c])) >> 32);
+                const uint32_t key = used[b] ? 0u : ((hiw & ~31u) | (uint32_t)l);
+                idx[b] = (int)(gmax32<K>(key) & 31u);
+            }
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                double* row = lds_row + (((c & 1) * 2 + b) * G + g) * NP;
+                if (l == idx[b]) {
+#pragma unroll
+                    for (int j = 0; j < NP; j++) row[j] = a[b][j];
+                    piv[(b * G + g) * NP + c] = idx[b];
+                }
             }
             __syncthreads();
-            const double ip = 1.0 / row[c];
-            if (l == idx) {
-                used = true;
-                mycol = c;
 #pragma unroll
-                for (int j = 0; j < NP; j++) a[j] = (j == c) ? ip : a[j] * ip;
-            } else {
-                const double f = a[c] * ip;
+            for (int b = 0; b < 2; b++) {
+                const double* row = lds_row + (((c & 1) * 2 + b) * G + g) * NP;
+                // reciprocal of the pivot: hardware estimate and one Newton step
+                const double pv = row[c];
+                double ip = __builtin_amdgcn_rcp(pv);
+                ip = fma(ip, fma(-pv, ip, 1.0), ip);
+                // one update for every lane: the pivot lane's row equals `row`,
+                // so a - (1 - ip) row scales it by ip; the others eliminate
+                // column c with f = a[c] ip
+                const bool me = l == idx[b];
+                const double f = me ? 1.0 - ip : a[b][c] * ip;
+                const double cv = me ? ip : -f;
+                if (me) {
+                    used[b] = true;
+                    mycol[b] = c;
+                }
 #pragma unroll
-                for (int j = 0; j < NP; j++) a[j] = (j == c) ? -f : a[j] - f * row[j];
+                for (int j = 0; j < NP; j++) a[b][j] = (j == c) ? cv : a[b][j] - f * row[j];
             }
         });
-        // lane rowof[c] now holds row c of the inverse with its columns permuted:
-        // a[j] = inv[c][rowof[j]].  Transpose through LDS so lane q gets inv[.][q].
+        // lane piv[c] now holds row c of the inverse with its columns permuted:
+        // a[j] = inv[c][piv[j]].  Transpose through LDS so lane q gets inv[.][q].
         double* T = lds_T + g * NP * NP;
-        if (l < NP) {
+        auto transpose = [&](int b, R(&out)[NP]) {
+            if (l < NP) {
 #pragma unroll
-            for (int j = 0; j < NP; j++) T[rowof[j] * NP + mycol] = a[j];
-        }
-        __syncthreads();
-        if (l < NP) {
+                for (int j = 0; j < NP; j++) T[piv[(b * G + g) * NP + j] * NP + mycol[b]] = a[b][j];
+            }
+            __syncthreads();
 #pragma unroll
-            for (int k = 0; k < NP; k++) out[k] = R(T[l * NP + k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < NP; k++) out[k] = R(0);
-        }
-        __syncthreads();
+            for (int k = 0; k < NP; k++) out[k] = l < NP ? R(T[l * NP + k]) : R(0);
+            __syncthreads();
+        };
+        transpose(0, out0);
+        transpose(1, out1);
     }
 
     __device__ __forceinline__ void rebuild_slater(const WalkParams& P, double* lds_T, double* lds_row)
     {
-        invert_block(P, 0, lds_T, lds_row, a0);
-        invert_block(P, 1, lds_T, lds_row, a1);
+        invert_blocks(P, lds_T, lds_row, a0, a1);
         act = 0;
         R gx, gy, sec;
         slater_derivs(P, x[0][0], x[0][1], a0, gx, gy, sec);
@@ -427,7 +456,7 @@ struct Walker {
                 const int q = s * NP + l;
                 if (!valid[s] || q == qq) continue;
                 const R e0 = x[s][0] - y0, e1 = x[s][1] - y1;
-                const R r = M<R>::sqrt(e0 * e0 + e1 * e1);
+                const R r = M<R>::fsqrt(e0 * e0 + e1 * e1);
                 const R ir = M<R>::rcp(r);
                 const R a = (upspin[s] == qq_up) ? R(P.a_same) : R(P.a_anti);
                 const R f = M<R>::rcp(R(1) + gam * r);
@@ -1426,7 +1455,7 @@ __device__ __forceinline__ WalkDyn walk_dyn(const WalkParams& P)
 }
 
 template <int NP, int K>
-constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 2 * (64 / K) * NP; }
+constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 5 * (64 / K) * NP; }
 
 // electron densities of the walker's current state (ref src/sampler.cpp:639-692;
 // r_i = |x_i|, ref Metropolis::calculateRadialVectorElement, metropolis.cpp:31-39):
@@ -1775,7 +1804,7 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
     using WK = Walker<R, NP, K, RB>;
     constexpr int G = WK::G;
     __shared__ double lds_T[G * NP * NP];
-    __shared__ double lds_row[2 * G * NP];
+    __shared__ double lds_row[5 * G * NP];
     extern __shared__ double rbm_dyn[];
     WK w;
     w.th_alpha = P.alpha;
