@@ -205,30 +205,39 @@ def rbmpj_workload(args, V):
 BATCH_DOTS = [(2, 1.0), (6, 1.0), (12, 1.0), (20, 0.5), (30, 0.5), (42, 0.28), (56, 0.1)]
 
 
+def batch_allotment(spec):
+    """Walkers per system: 'n2:T' = in proportion to N^2, T in total; an
+    integer = that many for every system."""
+    if spec.startswith("n2:"):
+        total = int(spec[3:])
+        w = np.array([N * N for N, _ in BATCH_DOTS], dtype=np.float64)
+        return [max(1, int(round(total * x / w.sum()))) for x in w]
+    return [int(spec)] * len(BATCH_DOTS)
+
+
 def batch_workload(args, V):
-    """value from the first of --batch-walkers (comma-separated walker counts
-    per system); every count is measured the same way."""
-    counts = [int(c) for c in str(args.batch_walkers).split(",")]
-    lines = [batch_measure(args, V, W) for W in counts]
+    """value from the first of --batch-walkers (comma-separated allotments:
+    'n2:T' or a count per system); every allotment is measured the same way."""
+    lines = [batch_measure(args, V, batch_allotment(c)) for c in str(args.batch_walkers).split(",")]
     out = lines[0]
     out["other_walker_counts"] = lines[1:]
     print(json.dumps(out))
 
 
-def batch_measure(args, V, W):
+def batch_measure(args, V, Ws):
     """Seven closed-shell dots N = 2..56 (Slater x Gaussian x Pade-Jastrow,
     Coulomb, alpha = gamma = 1, exact Langevin dt --dt with drift limit, FP32
-    walk with an FP64 rebuild every --refresh steps), --batch-walkers walkers
-    each.  One optimisation iteration = --batch-steps steps per walker of every
+    walk with an FP64 rebuild every --refresh steps), walkers per system Ws.  One optimisation iteration = --batch-steps steps per walker of every
     system, per-system sums and the device ADAM step (learning rate
     --batch-lr).  value = walker-steps/s of all systems together, one
-    persistent launch per refresh segment, the iteration replayed as a hipGraph,
+    kernel per system per refresh segment, the kernels side by side on streams
+    of their own, the iteration replayed as a hipGraph,
     shapes from the autotuner; beside it the same iterations with the default
     shapes, without the graph, and as seven single-system engines run one after
     another (host optimiser)."""
     S, lr = args.batch_steps, args.batch_lr
     systems = [dict(num_particles=N, omega=w, num_walkers=W, step_length=args.dt, alpha=1.0,
-                    gamma=1.0) for N, w in BATCH_DOTS]
+                    gamma=1.0) for (N, w), W in zip(BATCH_DOTS, Ws)]
     kinds = [V.SLATER, V.GAUSSIAN, V.PADE]
 
     def make_batch():
@@ -244,22 +253,22 @@ def batch_measure(args, V, W):
         wall = time.perf_counter() - t0
         return wall, ms / 1000.0, out
 
-    total_ws = W * len(systems) * S * args.steps
+    total_ws = sum(Ws) * S * args.steps
     res = {}
     b = make_batch()
     wall, kern, _ = time_batch(b, True)
     res["default_shapes_graph"] = {"walker_steps_per_s": total_ws / wall,
                                    "kernel_walker_steps_per_s": total_ws / kern}
     print(f"batch default shapes: {total_ws / wall:.4g} walker-steps/s", file=sys.stderr, flush=True)
-    lanes, waves, ns = b.autotune(args.batch_tune_steps)
-    print(f"autotune lanes={lanes.tolist()} waves={waves.tolist()}", file=sys.stderr, flush=True)
+    lanes, ns = b.autotune(args.batch_tune_steps)
+    print(f"autotune lanes={lanes.tolist()} ns={ns.tolist()}", file=sys.stderr, flush=True)
     wall, kern, out = time_batch(b, True)
     res["graph"] = {"walker_steps_per_s": total_ws / wall, "kernel_walker_steps_per_s": total_ws / kern,
                     "ms_per_step": 1000.0 * wall / args.steps}
     wall_d, kern_d, _ = time_batch(b, False)
     res["direct"] = {"walker_steps_per_s": total_ws / wall_d,
                      "kernel_walker_steps_per_s": total_ws / kern_d}
-    energies = (out[-1, :, 0] / (W * S)).tolist()
+    energies = [float(out[-1, s, 0] / (Ws[s] * S)) for s in range(len(systems))]
     params = b.parameters().tolist()
     b.close()
     print(f"batch autotuned graph: {res['graph']['walker_steps_per_s']:.4g}", file=sys.stderr,
@@ -271,7 +280,7 @@ def batch_measure(args, V, W):
     for s, (N, w) in enumerate(BATCH_DOTS):
         cfg = V.make_config(N, omega=w, kinds=kinds, sampling=V.IMPORTANCE_EXACT,
                             step_length=args.dt, quirks=0, drift_limit=args.drift_limit,
-                            precision=32, refresh_interval=args.refresh, num_walkers=W)
+                            precision=32, refresh_interval=args.refresh, num_walkers=Ws[s])
         e = V.Engine(cfg)
         P = np.array([[0.0], [1.0], [1.0]])
         e.set_parameters(P)
@@ -297,7 +306,7 @@ def batch_measure(args, V, W):
         rec[0].close()
     res["sequential_engines"] = {"walker_steps_per_s": total_ws / wall_s}
     value = res["graph"]["walker_steps_per_s"]
-    out = {"metric": "walker-steps/s, 7 quantum dots N=2..56 in one persistent launch",
+    out = {"metric": "walker-steps/s, 7 quantum dots N=2..56 walked side by side",
            "value": value, "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res["graph"]["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -306,10 +315,10 @@ def batch_measure(args, V, W):
            "config": {"workload": "closed-shell 2-D dots N=2,6,12,20,30,42,56 (omega "
                                   "1,1,1,0.5,0.5,0.28,0.1), Coulomb, Slater x Gaussian x "
                                   "Pade-Jastrow, exact Langevin",
-                      "walkers_per_system": W, "steps_per_iteration": S, "dt": args.dt,
+                      "walkers_per_system": Ws, "total_walkers": sum(Ws),
+                      "steps_per_iteration": S, "dt": args.dt,
                       "refresh_interval": args.refresh, "learning_rate": lr},
-           "autotune": {"lanes_per_walker": lanes.tolist(), "waves_per_tile": waves.tolist(),
-                        "ns_per_walker_step": ns.tolist()},
+           "autotune": {"lanes_per_walker": lanes.tolist(), "ns_per_walker_step": ns.tolist()},
            "forms": res,
            "speedup_vs_sequential_engines": value / res["sequential_engines"]["walker_steps_per_s"],
            "energies_last_iteration": energies, "alpha_gamma": params}
@@ -410,10 +419,11 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--workload", choices=["dot56", "rbmpj", "batch", "copy"], default="dot56",
                     help="dot56: the headline N=56 workload; rbmpj: RBM x Slater x Pade N=12 H=24; "
-                         "batch: seven dots N=2..56 in one persistent launch; "
+                         "batch: seven dots N=2..56 walked side by side; "
                          "copy: a plain device copy of --copy-mib MiB (calibrates HBM byte counters)")
-    ap.add_argument("--batch-walkers", default="8192,1024",
-                    help="walkers per system, comma-separated: one measurement each")
+    ap.add_argument("--batch-walkers", default="n2:131072,8192,1024",
+                    help="walker allotments, comma-separated, one measurement each: 'n2:T' "
+                         "(in proportion to N^2, T in total) or a count per system")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-lr", type=float, default=0.01)
     ap.add_argument("--batch-tune-steps", type=int, default=50)

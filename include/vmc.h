@@ -238,10 +238,11 @@ int vmc_lp_gradient(vmc_engine* e, const double* positions, int walkers, const i
 int vmc_lp_train(vmc_engine* e, int iterations, int walk_steps, double learning_rate,
                  double* history);
 
-/* ---- Batches: several quantum dots in one persistent launch ----
+/* ---- Batches: several quantum dots walked side by side ----
  * Systems of different particle numbers and frequencies (Slater / Gaussian /
- * Pade-Jastrow elements, the same for all) are walked by one launch whose
- * workgroups take tiles of walkers from a work queue, largest systems first.
+ * Pade-Jastrow elements, the same for all) are walked by one kernel each, the
+ * kernels launched together on streams of their own (costliest system first)
+ * so that the GPU's workgroup dispatcher fills the machine from all of them.
  * vmc_batch_run runs whole optimisation iterations on the device: the walk,
  * per-system sums, the reference's ADAM step on (alpha, gamma) of every system
  * (ref src/Optimization/adam.cpp:12-38) and the advance of the random counter
@@ -282,14 +283,13 @@ int vmc_batch_get_positions(vmc_batch* b, int system, double* positions, int wal
  * system after the last vmc_batch_run (the device optimiser's values). */
 int vmc_batch_get_parameters(vmc_batch* b, double* alpha_gamma);
 /* Kernel shape of one system: lanes per walker (a power of two >= the
- * orbitals per spin) and waves per tile. */
-int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker, int waves_per_tile);
+ * orbitals per spin, at most twice the smallest such). */
+int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker);
 /* Times each system's candidate shapes on its own walkers (positions and
- * random state are restored) and keeps the fastest; lanes_out / waves_out
- * [num_systems] receive the choices, ns_per_step_out [num_systems] (may be
- * null) their time per walker-step. */
-int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, int* waves_out,
-                       double* ns_per_step_out);
+ * random state are restored) and keeps the fastest; lanes_out [num_systems]
+ * receives the choices, ns_per_step_out [num_systems] (may be null) their time
+ * per walker-step, which then orders the launches. */
+int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, double* ns_per_step_out);
 /* `iterations` optimisation iterations of equilibration + steps steps each.
  * out [iterations][num_systems][VMC_BATCH_OUT]; kernel_ms (may be null) = the
  * device time of all iterations.  learning_rate 0 keeps the parameters. */

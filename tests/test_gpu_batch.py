@@ -1,13 +1,14 @@
-"""The batch form: quantum dots of different electron counts walked by one
-persistent launch, an optimisation iteration per hipGraph replay, the
-optimiser on the device (vmc_batch_* in include/vmc.h).
+"""The batch form: quantum dots of different electron counts walked side by
+side (one kernel per system on streams of their own), an optimisation
+iteration per hipGraph replay, the optimiser on the device (vmc_batch_* in
+include/vmc.h).
 
 Pinned three ways:
  * every system of a batch retraces the FP64 oracle's chains (Philox stream
    seed + s for system s) step for step, as a single-system engine does;
  * per system, the batch's sums and final positions equal those of a
    single-system engine seeded seed + s, in FP64 and FP32, also after the
-   kernel shape (lanes per walker, waves per tile) is changed;
+   kernel shape (lanes per walker) is changed;
  * graph replay and direct launches give bit-identical results, and the
    device ADAM step reproduces the host optimiser (ref
    src/Optimization/adam.cpp:12-38) iteration by iteration.
@@ -194,7 +195,7 @@ def test_device_adam_matches_host_optimiser():
 
 
 def test_shapes_and_autotune():
-    """A changed shape (2K lanes, 2 waves per tile) walks the same chains; the
+    """A changed shape (2K lanes per walker) walks the same chains; the
     autotuner picks valid shapes and leaves positions untouched."""
     V = _require_gpu()
     rng = np.random.default_rng(3)
@@ -207,7 +208,7 @@ def test_shapes_and_autotune():
     with _batch(V, systems, O.IMPORTANCE_EXACT) as b:
         for s, y in enumerate(systems):
             K = LANES[ORBITALS[y["num_particles"]]]
-            b.set_shape(s, 2 * K, 2)
+            b.set_shape(s, 2 * K)
         b.init_walkers(seed)
         out, _ = b.run(1, steps)
         for s in range(len(systems)):
@@ -216,11 +217,11 @@ def test_shapes_and_autotune():
     with _batch(V, systems, O.IMPORTANCE_EXACT, precision=32) as b:
         b.init_walkers(seed)
         before = [b.get_positions(s) for s in range(len(systems))]
-        lanes, waves, ns = b.autotune(30)
-        print("autotune", lanes, waves, ns, flush=True)
+        lanes, ns = b.autotune(30)
+        print("autotune", lanes, ns, flush=True)
         for s, y in enumerate(systems):
             K = LANES[ORBITALS[y["num_particles"]]]
-            assert lanes[s] in (K, 2 * K) and waves[s] in (1, 2, 4) and ns[s] > 0
+            assert lanes[s] in (K, 2 * K) and ns[s] > 0
             np.testing.assert_array_equal(b.get_positions(s), before[s])
         out, _ = b.run(2, 100, 10)
         assert np.all(np.isfinite(out))

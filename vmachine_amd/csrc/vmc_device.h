@@ -145,15 +145,14 @@ struct EvalOut {
     int num_elements, maxpar;
 };
 
-// ---- batch of systems walked by one persistent launch (vmc_batch.hip) ----
+// ---- batch of systems walked side by side (vmc_batch.hip) ----
 enum : int { kBatchAlpha = kNumWalkerAcc, kBatchGamma, kBatchOut };
 
 struct BatchSystem {
     WalkParams P;       // the system's parameters and buffers; per-iteration fields set in-kernel
     int shape;          // instantiation id (np, K)
     int np, K;
-    int waves;          // blocks (waves) per tile
-    int tile0, ntiles;  // the system's tiles in the work queue
+    int blocks;         // one-wave workgroups of the system's kernel
     int walker0;        // first row of the system in the shared walker buffers
     int user;           // the system's index in the caller's list
 };
@@ -162,8 +161,8 @@ struct BatchSystem {
 // by the host before a run, read-only for the kernels)
 constexpr int kBatchMaxSystems = 8;
 struct BatchTable {
-    BatchSystem sys[kBatchMaxSystems];   // in tile order (costliest first)
-    int nsys, total_tiles;
+    BatchSystem sys[kBatchMaxSystems];   // launch order: costliest system first
+    int nsys;
 };
 
 struct BatchState {

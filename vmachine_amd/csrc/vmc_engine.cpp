@@ -34,15 +34,13 @@ hipError_t launch_lp_adam(float* theta, uint16_t* bf16, float* m, float* v, cons
 int lp_wgrad_rows(int B);
 int lp_act_floats();
 int lp_delta_floats();
-hipError_t launch_batch_walk(int precision, const BatchTable* T, int* queue, int grid, size_t lds,
-                             int t_begin, int t_end, int acc_add, const BatchState* st,
-                             const double* theta, hipStream_t stream);
+hipError_t launch_batch_walk(int precision, int shape, const WalkParams& P, int user, int t_begin,
+                             int t_end, int acc_add, const BatchState* st, const double* theta,
+                             hipStream_t stream);
 hipError_t launch_batch_tail(const BatchTable* T, int nsys, BatchOpt* opt, double* theta,
                              const double* acc, double* out, BatchState* st, int it0,
                              int steps_total, hipStream_t stream);
 int batch_shape_id(int np, int K);
-size_t batch_lds_bytes(int np, int K);
-int batch_resident_per_cu(int precision, size_t lds);
 // out: [1 + ceil(W/256)][kNumWalkerAcc] (the sums, then block partials)
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
@@ -993,76 +991,85 @@ int vmc_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, 
 struct vmc_batch {
     vmc_batch_config cfg{};
     std::vector<vmc_batch_system> sys;     // caller order
-    std::vector<vmc::BatchSystem> hs;      // tile order (costliest first)
-    std::vector<int> dev_of;               // caller index -> tile-order index
-    std::vector<double> theta;             // [tile order][alpha, gamma]
-    int total_walkers = 0, total_tiles = 0, grid = 0, cus = 0;
-    size_t lds = 0;
+    std::vector<vmc::BatchSystem> hs;      // table order: costliest first
+    std::vector<int> dev_of;               // caller index -> table index
+    std::vector<double> cost;              // caller order: time per walker-step (relative)
+    std::vector<double> theta;             // caller order: alpha, gamma
+    int total_walkers = 0;
     uint64_t seed = 0, counter = 0;
     int iteration = 0;
     bool walkers_set = false, fresh = true;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side[vmc::kBatchMaxSystems] = {};   // one more stream per further system
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
+    hipEvent_t join[vmc::kBatchMaxSystems] = {};
     vmc::BatchTable* d_table = nullptr;    // [2]: the run's table, the autotuner's
     vmc::BatchState* d_st = nullptr;
-    vmc::BatchOpt* d_opt = nullptr;
+    vmc::BatchOpt* d_opt = nullptr;        // caller order
     double *d_theta = nullptr, *d_pos = nullptr, *d_qf = nullptr, *d_acc = nullptr;
-    int* d_queue = nullptr;
     std::vector<size_t> pos_off;           // caller index -> first double of its positions
 
     ~vmc_batch()
     {
-        for (void* p : {(void*)d_table, (void*)d_st, (void*)d_opt, (void*)d_theta, (void*)d_pos, (void*)d_qf,
-                        (void*)d_acc, (void*)d_queue})
+        for (void* p : {(void*)d_table, (void*)d_st, (void*)d_opt, (void*)d_theta, (void*)d_pos,
+                        (void*)d_qf, (void*)d_acc})
             if (p) (void)hipFree(p);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : {ev0, ev1, fork})
+            if (e) (void)hipEventDestroy(e);
+        for (int i = 0; i < vmc::kBatchMaxSystems; i++) {
+            if (join[i]) (void)hipEventDestroy(join[i]);
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
 namespace {
 
-int batch_tiles(const vmc::BatchSystem& B)
-{
-    const int G = 64 / B.K;
-    return ((B.P.W + G - 1) / G + B.waves - 1) / B.waves;
-}
-
-// tiles of every system for its current shape; LDS and the persistent grid
+// the launch order for the systems' current shapes: the costliest system
+// (walkers x time per walker-step, measured or modelled) first, so that the
+// longest kernel starts first and the shorter ones fill in beside it
 void batch_plan(vmc_batch* b)
 {
-    int t = 0;
-    size_t lds = 0;
-    for (auto& B : b->hs) {
-        B.ntiles = batch_tiles(B);
-        B.tile0 = t;
-        t += B.ntiles;
-        lds = std::max(lds, vmc::batch_lds_bytes(B.np, B.K));
+    auto work = [&](const vmc::BatchSystem& B) { return B.P.W * b->cost[B.user]; };
+    std::stable_sort(b->hs.begin(), b->hs.end(),
+                     [&](const vmc::BatchSystem& x, const vmc::BatchSystem& y) { return work(x) > work(y); });
+    for (int d = 0; d < (int)b->hs.size(); d++) {
+        vmc::BatchSystem& B = b->hs[d];
+        b->dev_of[B.user] = d;
+        B.blocks = (B.P.W + 64 / B.K - 1) / (64 / B.K);
     }
-    b->total_tiles = t;
-    b->lds = lds;
-    b->grid = b->cus * std::max(1, vmc::batch_resident_per_cu(b->cfg.precision, lds));
 }
 
 vmc::BatchTable batch_table(const vmc_batch* b)
 {
     vmc::BatchTable T{};
     T.nsys = (int)b->hs.size();
-    T.total_tiles = b->total_tiles;
-    for (int d = 0; d < T.nsys; d++) T.sys[d] = b->hs[d];
+    for (int d = 0; d < T.nsys; d++) {
+        T.sys[d] = b->hs[d];
+    }
     return T;
 }
 
-// one iteration's walk launches (one per refresh segment) over the table in
-// d_table[0]
+// one iteration's walk: per refresh segment the systems' kernels side by
+// side (the main stream forks to one stream per further system and joins
+// them again)
 int batch_enqueue_walk(vmc_batch* b, int total)
 {
     const int seg = b->cfg.refresh_interval > 0 ? b->cfg.refresh_interval : total;
+    const int n = (int)b->hs.size();
     for (int t0 = 0; t0 < total; t0 += seg) {
         const int t1 = t0 + seg < total ? t0 + seg : total;
-        HIP_TRY(vmc::launch_batch_walk(b->cfg.precision, b->d_table, b->d_queue, b->grid, b->lds,
-                                       t0, t1, t0 > 0 ? 1 : 0, b->d_st, b->d_theta, b->stream));
+        if (n > 1) HIP_TRY(hipEventRecord(b->fork, b->stream));
+        for (int d = 0; d < n; d++) {
+            const vmc::BatchSystem& B = b->hs[d];
+            hipStream_t st = d == 0 ? b->stream : b->side[d - 1];
+            if (d > 0) HIP_TRY(hipStreamWaitEvent(st, b->fork, 0));
+            HIP_TRY(vmc::launch_batch_walk(b->cfg.precision, B.shape, B.P, B.user, t0, t1,
+                                           t0 > 0 ? 1 : 0, b->d_st, b->d_theta, st));
+            if (d > 0) HIP_TRY(hipEventRecord(b->join[d - 1], st));
+        }
+        for (int d = 1; d < n; d++) HIP_TRY(hipStreamWaitEvent(b->stream, b->join[d - 1], 0));
     }
     return VMC_OK;
 }
@@ -1101,7 +1108,6 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
     b->cfg = c;
     b->sys.assign(c.systems, c.systems + c.num_systems);
     b->cfg.systems = nullptr;
-    std::vector<double> cost(c.num_systems);
     std::vector<vmc::BatchSystem> hs(c.num_systems);
     size_t pos = 0, acc = 0;
     b->pos_off.resize(c.num_systems);
@@ -1153,13 +1159,11 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         B.np = np;
         B.K = K;
         B.shape = vmc::batch_shape_id(np, K);
-        B.waves = 1;
         B.user = s;
         B.walker0 = (int)acc;
         b->pos_off[s] = pos;
         pos += (size_t)y.num_walkers * 2 * y.num_particles;
         acc += (size_t)y.num_walkers;
-        cost[s] = double(y.num_particles) * y.num_particles * y.num_walkers;
     }
     b->total_walkers = (int)acc;
     int ndev = 0;
@@ -1168,7 +1172,6 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
     hipError_t err;
     int dev = 0;
     if ((err = hipGetDevice(&dev)) != hipSuccess ||
-        (err = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
         (err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
         (err = hipEventCreate(&b->ev0)) != hipSuccess || (err = hipEventCreate(&b->ev1)) != hipSuccess ||
         (err = hipMalloc(&b->d_pos, sizeof(double) * pos)) != hipSuccess ||
@@ -1178,25 +1181,26 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         (err = hipMalloc(&b->d_table, 2 * sizeof(vmc::BatchTable))) != hipSuccess ||
         (err = hipMalloc(&b->d_st, sizeof(vmc::BatchState))) != hipSuccess ||
         (err = hipMalloc(&b->d_opt, sizeof(vmc::BatchOpt) * c.num_systems)) != hipSuccess ||
-        (err = hipMalloc(&b->d_queue, sizeof(int))) != hipSuccess)
+        (err = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess)
         return cleanup(fail(VMC_E_DEVICE, std::string("batch set-up: ") + hipGetErrorString(err)));
+    for (int i = 0; i + 1 < c.num_systems; i++)
+        if ((err = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking)) != hipSuccess ||
+            (err = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming)) != hipSuccess)
+            return cleanup(fail(VMC_E_DEVICE, std::string("batch set-up: ") + hipGetErrorString(err)));
     for (int s = 0; s < c.num_systems; s++) {
         vmc::WalkParams& P = hs[s].P;
         P.pos = b->d_pos + b->pos_off[s];
         P.qf = (c.sampling == VMC_IMPORTANCE) ? b->d_qf + b->pos_off[s] : nullptr;
         P.acc = b->d_acc + (size_t)hs[s].walker0 * vmc::kNumWalkerAcc;
+        b->theta.push_back(hs[s].P.alpha);
+        b->theta.push_back(hs[s].P.gamma);
     }
-    // tile order: costliest systems first, so that the small ones fill the tail
-    std::vector<int> order(c.num_systems);
-    for (int s = 0; s < c.num_systems; s++) order[s] = s;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    b->dev_of.resize(c.num_systems);
-    for (int i = 0; i < c.num_systems; i++) {
-        b->hs.push_back(hs[order[i]]);
-        b->dev_of[order[i]] = i;
-        b->theta.push_back(hs[order[i]].P.alpha);
-        b->theta.push_back(hs[order[i]].P.gamma);
-    }
+    // the cost model until the autotuner measures: time per walker-step ~ N^2
+    b->hs = hs;
+    b->dev_of.assign(c.num_systems, 0);
+    b->cost.resize(c.num_systems);
+    for (int s = 0; s < c.num_systems; s++)
+        b->cost[s] = double(b->sys[s].num_particles) * b->sys[s].num_particles;
     batch_plan(b);
     // ADAM moments start at one (ref src/Optimization/adam.cpp:17-19)
     std::vector<vmc::BatchOpt> opt(c.num_systems, vmc::BatchOpt{{1.0, 1.0}, {1.0, 1.0}, 0});
@@ -1258,24 +1262,19 @@ int vmc_batch_get_positions(vmc_batch* b, int system, double* positions, int wal
 int vmc_batch_get_parameters(vmc_batch* b, double* alpha_gamma)
 {
     if (!b || !alpha_gamma) return fail(VMC_E_INVALID, "null argument");
-    for (size_t d = 0; d < b->hs.size(); d++) {
-        alpha_gamma[2 * b->hs[d].user] = b->theta[2 * d];
-        alpha_gamma[2 * b->hs[d].user + 1] = b->theta[2 * d + 1];
-    }
+    for (size_t i = 0; i < b->theta.size(); i++) alpha_gamma[i] = b->theta[i];
     return VMC_OK;
 }
 
-int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker, int waves_per_tile)
+int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker)
 {
     if (!b) return fail(VMC_E_INVALID, "null batch");
     if (system < 0 || system >= (int)b->sys.size()) return fail(VMC_E_INVALID, "no such system");
     vmc::BatchSystem& B = b->hs[b->dev_of[system]];
     const int id = vmc::batch_shape_id(B.np, lanes_per_walker);
     if (id < 0) return fail(VMC_E_INVALID, "no batch kernel shape for these lanes per walker");
-    if (waves_per_tile < 1 || waves_per_tile > 64) return fail(VMC_E_INVALID, "waves_per_tile in 1..64");
     B.K = lanes_per_walker;
     B.shape = id;
-    B.waves = waves_per_tile;
     batch_plan(b);
     return VMC_OK;
 }
@@ -1351,26 +1350,18 @@ int vmc_batch_run(vmc_batch* b, int iterations, int steps, int equilibration, in
     b->counter += (uint64_t)iterations * total;
     b->iteration += iterations;
     b->fresh = false;
-    if (out)
-        for (int it = 0; it < iterations; it++)
-            for (int d = 0; d < nsys; d++) {
-                const int u = b->hs[d].user;
-                for (int k = 0; k < vmc::kBatchOut; k++)
-                    out[((size_t)it * nsys + u) * vmc::kBatchOut + k] =
-                        h[((size_t)it * nsys + d) * vmc::kBatchOut + k];
-            }
+    if (out) std::copy(h.begin(), h.end(), out);   // [iteration][caller's system][16]
     if (kernel_ms) *kernel_ms = ms;
     return done(VMC_OK);
 }
 
-int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, int* waves_out,
-                       double* ns_per_step_out)
+int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, double* ns_per_step_out)
 {
     if (!b) return fail(VMC_E_INVALID, "null batch");
     if (!b->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
     if (steps < 1) return fail(VMC_E_INVALID, "steps >= 1");
     const int nsys = (int)b->hs.size();
-    // each system alone in a one-entry table, walked from its saved positions
+    // each system alone on the whole GPU, walked from its saved positions
     // and drifts, which are restored after every candidate
     size_t maxpos = 0;
     for (const auto& y : b->sys) maxpos = std::max(maxpos, (size_t)y.num_walkers * 2 * y.num_particles);
@@ -1384,7 +1375,7 @@ int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, int* waves_out,
     const vmc::BatchState st{b->counter, 0, 1, b->fresh ? 1 : 0, 0.0};
     if ((err = hipMemcpy(b->d_st, &st, sizeof(st), hipMemcpyHostToDevice)) != hipSuccess)
         return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
-    for (int d = 0; d < nsys; d++) {
+    for (int d = 0; d < nsys; d++) {   // table order is kept until the re-plan below
         const vmc::BatchSystem B = b->hs[d];
         const size_t n = (size_t)B.P.W * B.P.ND;
         const size_t off = b->pos_off[B.user];
@@ -1392,53 +1383,35 @@ int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, int* waves_out,
             (err = hipMemcpy(d_save + n, b->d_qf + off, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess)
             return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
         double best = 1e300;
-        int bestK = B.K, bestW = B.waves;
+        int bestK = B.K;
         for (int K : {B.K, 2 * B.K}) {
             const int id = vmc::batch_shape_id(B.np, K);
             if (id < 0) continue;
-            for (int waves : {1, 2, 4}) {
-                vmc::BatchTable T{};
-                T.nsys = 1;
-                vmc::BatchSystem& C = T.sys[0];
-                C = B;
-                C.K = K;
-                C.shape = id;
-                C.waves = waves;
-                C.tile0 = 0;
-                C.ntiles = batch_tiles(C);
-                C.P.seed = b->seed + (uint64_t)B.user;
-                C.P.steps = steps;
-                C.P.equil = 0;
-                T.total_tiles = C.ntiles;
-                const size_t lds = vmc::batch_lds_bytes(C.np, K);
-                const int grid = std::min(
-                    b->cus * std::max(1, vmc::batch_resident_per_cu(b->cfg.precision, lds)), C.ntiles);
-                float ms = 0;
-                for (int rep = 0; rep < 2; rep++)   // the second launch is timed
-                    if ((err = hipMemcpy(b->d_table + 1, &T, sizeof(T), hipMemcpyHostToDevice)) != hipSuccess ||
-                        (err = hipEventRecord(b->ev0, b->stream)) != hipSuccess ||
-                        (err = vmc::launch_batch_walk(b->cfg.precision, b->d_table + 1, b->d_queue, grid, lds, 0,
-                                                      steps, 0, b->d_st, b->d_theta + 2 * d,
-                                                      b->stream)) != hipSuccess ||
-                        (err = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
-                        (err = hipStreamSynchronize(b->stream)) != hipSuccess ||
-                        (err = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess ||
-                        (err = hipMemcpy(b->d_pos + off, d_save, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess ||
-                        (err = hipMemcpy(b->d_qf + off, d_save + n, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess)
-                        return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
-                const double ns = 1e6 * ms / (double(C.P.W) * steps);
-                if (ns < best) {
-                    best = ns;
-                    bestK = K;
-                    bestW = waves;
-                }
+            vmc::WalkParams P = B.P;
+            P.seed = b->seed + (uint64_t)B.user;
+            P.steps = steps;
+            P.equil = 0;
+            float ms = 0;
+            for (int rep = 0; rep < 2; rep++)   // the second launch is timed
+                if ((err = hipEventRecord(b->ev0, b->stream)) != hipSuccess ||
+                    (err = vmc::launch_batch_walk(b->cfg.precision, id, P, B.user, 0, steps, 0,
+                                                  b->d_st, b->d_theta, b->stream)) != hipSuccess ||
+                    (err = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
+                    (err = hipStreamSynchronize(b->stream)) != hipSuccess ||
+                    (err = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess ||
+                    (err = hipMemcpy(b->d_pos + off, d_save, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess ||
+                    (err = hipMemcpy(b->d_qf + off, d_save + n, sizeof(double) * n, hipMemcpyDeviceToDevice)) != hipSuccess)
+                    return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+            const double ns = 1e6 * ms / (double(P.W) * steps);
+            if (ns < best) {
+                best = ns;
+                bestK = K;
             }
         }
         b->hs[d].K = bestK;
         b->hs[d].shape = vmc::batch_shape_id(B.np, bestK);
-        b->hs[d].waves = bestW;
+        b->cost[B.user] = best;
         if (lanes_out) lanes_out[B.user] = bestK;
-        if (waves_out) waves_out[B.user] = bestW;
         if (ns_per_step_out) ns_per_step_out[B.user] = best;
     }
     batch_plan(b);

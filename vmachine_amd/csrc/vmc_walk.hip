@@ -339,19 +339,22 @@ struct Walker {
         sec = s * s * ss;
     }
 
-    // FP64 Gauss-Jordan inverses of the two spin blocks with partial
-    // pivoting, the pivot chains of the two blocks interleaved; leaves column l
-    // of the inverse of block 0 in out0 and of block 1 in out1.  Lane l holds
-    // row l of each block (orbital values of particle b NP + l).  lds_row:
-    // [2 buffers][2 blocks][G][NP] pivot rows, then [2][G][NP] pivot lanes as int.
-    __device__ __forceinline__ void invert_blocks(const WalkParams& P, double* lds_T,
+    // FP64 Gauss-Jordan inverses of NBK spin blocks (b0, b0 + 1, ...) with
+    // partial pivoting, the pivot chains of the blocks interleaved; leaves
+    // column l of the inverse of block b0 in out0 (and of block 1 in out1 when
+    // NBK = 2).  Lane l holds row l of each block (orbital values of particle
+    // b NP + l).  lds_row: [2 buffers][2 blocks][G][NP] pivot rows, then
+    // [2][G][NP] pivot lanes as int.
+    template <int NBK>
+    __device__ __forceinline__ void invert_blocks(const WalkParams& P, int b0, double* lds_T,
                                                   double* lds_row, R (&out0)[NP], R (&out1)[NP])
     {
         const double s = P.sqrt_omega;
-        double a[2][NP];
+        double a[NBK][NP];
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const double t0 = s * double(x[b][0]), t1 = s * double(x[b][1]);
+        for (int b = 0; b < NBK; b++) {
+            const double t0 = s * double(vsel(b0 + b != 0, x[1][0], x[0][0]));
+            const double t1 = s * double(vsel(b0 + b != 0, x[1][1], x[0][1]));
             double hx[S + 1], hy[S + 1];
             hermite<double, S>(t0, hx);
             hermite<double, S>(t1, hy);
@@ -359,34 +362,39 @@ struct Walker {
             for (int k = 0; k < NP; k++) a[b][k] = hx[orb_nx(k)] * hy[orb_ny(k)];
         }
         int* piv = reinterpret_cast<int*>(lds_row + 4 * G * NP);   // [2][G][NP]
-        bool used[2] = {l >= NP, l >= NP};
-        int mycol[2] = {0, 0};
+        bool used[NBK];
+        int mycol[NBK];
+#pragma unroll
+        for (int b = 0; b < NBK; b++) {
+            used[b] = l >= NP;
+            mycol[b] = 0;
+        }
         sfor<0, NP>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
             // the pivot: the group maximum of one 32-bit key per lane, the high
             // word of |a| (sign, exponent and 20 mantissa bits, which order as
             // the value) with the low 5 bits replaced by the lane, found with
             // DPP steps; magnitudes within 2^-15 of each other count as equal
-            int idx[2];
+            int idx[NBK];
 #pragma unroll
-            for (int b = 0; b < 2; b++) {
+            for (int b = 0; b < NBK; b++) {
                 const uint32_t hiw = (uint32_t)(__builtin_bit_cast(uint64_t, fabs(a[b][c])) >> 32);
                 const uint32_t key = used[b] ? 0u : ((hiw & ~31u) | (uint32_t)l);
                 idx[b] = (int)(gmax32<K>(key) & 31u);
             }
 #pragma unroll
-            for (int b = 0; b < 2; b++) {
-                double* row = lds_row + (((c & 1) * 2 + b) * G + g) * NP;
+            for (int b = 0; b < NBK; b++) {
+                double* row = lds_row + (((c & 1) * 2 + b0 + b) * G + g) * NP;
                 if (l == idx[b]) {
 #pragma unroll
                     for (int j = 0; j < NP; j++) row[j] = a[b][j];
-                    piv[(b * G + g) * NP + c] = idx[b];
+                    piv[((b0 + b) * G + g) * NP + c] = idx[b];
                 }
             }
             __syncthreads();
 #pragma unroll
-            for (int b = 0; b < 2; b++) {
-                const double* row = lds_row + (((c & 1) * 2 + b) * G + g) * NP;
+            for (int b = 0; b < NBK; b++) {
+                const double* row = lds_row + (((c & 1) * 2 + b0 + b) * G + g) * NP;
                 // reciprocal of the pivot: hardware estimate and one Newton step
                 const double pv = row[c];
                 double ip = __builtin_amdgcn_rcp(pv);
@@ -411,7 +419,8 @@ struct Walker {
         auto transpose = [&](int b, R(&out)[NP]) {
             if (l < NP) {
 #pragma unroll
-                for (int j = 0; j < NP; j++) T[piv[(b * G + g) * NP + j] * NP + mycol[b]] = a[b][j];
+                for (int j = 0; j < NP; j++)
+                    T[piv[((b0 + b) * G + g) * NP + j] * NP + mycol[b]] = a[b][j];
             }
             __syncthreads();
 #pragma unroll
@@ -419,12 +428,12 @@ struct Walker {
             __syncthreads();
         };
         transpose(0, out0);
-        transpose(1, out1);
+        if constexpr (NBK == 2) transpose(1, out1);
     }
 
     __device__ __forceinline__ void rebuild_slater(const WalkParams& P, double* lds_T, double* lds_row)
     {
-        invert_blocks(P, lds_T, lds_row, a0, a1);
+        invert_blocks<2>(P, 0, lds_T, lds_row, a0, a1);
         act = 0;
         R gx, gy, sec;
         slater_derivs(P, x[0][0], x[0][1], a0, gx, gy, sec);
@@ -441,6 +450,7 @@ struct Walker {
     __device__ __forceinline__ void rebuild_pairs(const WalkParams& P)
     {
         const R gam = R(th_gamma);
+        const R a_same = R(P.a_same), a_anti = R(P.a_anti);
         double vi = 0, ljs = 0, jgs = 0;
 #pragma unroll
         for (int s = 0; s < 2; s++) gJ[s][0] = gJ[s][1] = R(0);
@@ -458,7 +468,7 @@ struct Walker {
                 const R e0 = x[s][0] - y0, e1 = x[s][1] - y1;
                 const R r = M<R>::fsqrt(e0 * e0 + e1 * e1);
                 const R ir = M<R>::rcp(r);
-                const R a = (upspin[s] == qq_up) ? R(P.a_same) : R(P.a_anti);
+                const R a = (upspin[s] == qq_up) ? a_same : a_anti;
                 const R f = M<R>::rcp(R(1) + gam * r);
                 const R u1 = a * f * f;
                 if (P.has_pade) {

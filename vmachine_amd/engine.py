@@ -128,8 +128,8 @@ def lib():
         for n in ["vmc_batch_set_positions", "vmc_batch_get_positions"]:
             getattr(L, n).argtypes = [ctypes.c_void_p, _I, _DP, _I, _I]
         L.vmc_batch_get_parameters.argtypes = [ctypes.c_void_p, _DP]
-        L.vmc_batch_set_shape.argtypes = [ctypes.c_void_p, _I, _I, _I]
-        L.vmc_batch_autotune.argtypes = [ctypes.c_void_p, _I, _IP, _IP, _DP]
+        L.vmc_batch_set_shape.argtypes = [ctypes.c_void_p, _I, _I]
+        L.vmc_batch_autotune.argtypes = [ctypes.c_void_p, _I, _IP, _DP]
         L.vmc_batch_run.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, _D, _I, _DP, _DP]
         _lib = L
     return _lib
@@ -357,7 +357,7 @@ class Engine:
 
 
 class Batch:
-    """Several systems walked by one persistent launch (vmc_batch_* in include/vmc.h).
+    """Several systems walked side by side, one kernel each (vmc_batch_* in include/vmc.h).
 
     systems: list of dicts with num_particles, omega, num_walkers, step_length,
     alpha, gamma.  System s draws from seed + s and retraces an Engine of that
@@ -435,16 +435,15 @@ class Batch:
         self._check(self.L.vmc_batch_get_parameters(self.h, _dp(ag)))
         return ag
 
-    def set_shape(self, s, lanes_per_walker, waves_per_tile):
-        self._check(self.L.vmc_batch_set_shape(self.h, s, lanes_per_walker, waves_per_tile))
+    def set_shape(self, s, lanes_per_walker):
+        self._check(self.L.vmc_batch_set_shape(self.h, s, lanes_per_walker))
 
     def autotune(self, steps):
+        """(lanes per walker, ns per walker-step) chosen for every system."""
         lanes = np.zeros(self.n, dtype=np.int32)
-        waves = np.zeros(self.n, dtype=np.int32)
         ns = np.zeros(self.n)
-        self._check(self.L.vmc_batch_autotune(self.h, steps, lanes.ctypes.data_as(_IP),
-                                              waves.ctypes.data_as(_IP), _dp(ns)))
-        return lanes, waves, ns
+        self._check(self.L.vmc_batch_autotune(self.h, steps, lanes.ctypes.data_as(_IP), _dp(ns)))
+        return lanes, ns
 
     def run(self, iterations, steps, equilibration=0, batches=1, learning_rate=0.0,
             use_graph=True):
