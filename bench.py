@@ -163,7 +163,9 @@ def cpu_baseline(args):
 def rbmpj_workload(args, V):
     """RBM x Slater x Pade-Jastrow, N=12, H=24, sigma=1, Xavier weights from the
     seed (ref src/InitialWeights/xavier.cpp), reference importance sampling
-    dt=0.01, 65536 walkers: walker-steps/s of the FP64 and FP32 walks."""
+    dt=0.01, 65536 walkers: walker-steps/s of the FP64 walk and of the FP32 walk
+    with each RBM-product contraction mode (value: the default, matrix cores in
+    FP32)."""
     N, H, nd = 12, 24, 24
     rng = np.random.default_rng(args.seed)
     a = 1.0 / np.sqrt(nd)
@@ -172,24 +174,29 @@ def rbmpj_workload(args, V):
     P[2, :] = rng.uniform(-a, a, H + nd * H)
     P[3, 0] = rng.uniform(0.0, 0.2)
     res = {}
-    for prec in (32, 64):
+    forms = [("fp32", 32, V.RBM_AUTO), ("fp32_lanes", 32, V.RBM_LANES),
+             ("fp32_mfma_bf16", 32, V.RBM_MFMA_BF16), ("fp64", 64, V.RBM_AUTO)]
+    for name, prec, mode in forms:
         cfg = V.make_config(N, omega=1.0, kinds=[V.SLATER, V.RBM_GAUSSIAN, V.RBM_PRODUCT, V.PADE],
                             sampling=V.IMPORTANCE, step_length=0.01, precision=prec,
                             refresh_interval=args.refresh if prec == 32 else 0,
-                            num_walkers=args.walkers, num_hidden=H, sigma=1.0, sorting=True)
+                            num_walkers=args.walkers, num_hidden=H, sigma=1.0, sorting=True,
+                            rbm_matrix=mode)
         eng = V.Engine(cfg)
         eng.set_parameters(P)
         eng.init_walkers(args.seed)
-        burn_in(eng, args.burnin, f"rbmpj fp{prec}")
+        burn_in(eng, args.burnin, f"rbmpj {name}")
         iters = args.steps if prec == 32 else args.fp64_iters
         wall, kern, E, acc = timed(eng, iters, args.warmup, args.steps_per_iter)
         ws = args.walkers * args.steps_per_iter * iters
-        res[prec] = {"walker_steps_per_s": ws / wall, "kernel_walker_steps_per_s": ws / kern,
+        res[name] = {"walker_steps_per_s": ws / wall, "kernel_walker_steps_per_s": ws / kern,
                      "energy": E, "acceptance": acc, "ms_per_step": 1000.0 * wall / iters}
+        print(f"rbmpj {name}: {ws / kern:.4g} walker-steps/s (kernel), E {E:.6f}",
+              file=sys.stderr, flush=True)
         eng.close()
     out = {"metric": "walker-steps/s, RBM x Slater x Pade-Jastrow N=12 H=24",
-           "value": res[32]["walker_steps_per_s"], "unit": "walker-steps/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res[32]["ms_per_step"],
+           "value": res["fp32"]["walker_steps_per_s"], "unit": "walker-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_step"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 walk, f64 inverse rebuild and accumulation",
            "data": "synthetic (walkers drawn from N(0, 1/omega^2), burned in; Xavier weights)",
@@ -197,8 +204,9 @@ def rbmpj_workload(args, V):
                                   "RBM-product (H=24, sorted) x Pade-Jastrow, reference "
                                   "importance sampling dt=0.01",
                       "walkers": args.walkers, "steps_per_iter": args.steps_per_iter,
-                      "refresh_interval": args.refresh, "burnin_steps": args.burnin},
-           "fp32": res[32], "fp64": res[64]}
+                      "refresh_interval": args.refresh, "burnin_steps": args.burnin,
+                      "rbm_contractions": "FP32 matrix-core tiles (V_MFMA_F32_16X16X4_F32)"},
+           "forms": res}
     print(json.dumps(out))
 
 
@@ -392,7 +400,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10, help="timed sampling iterations")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--steps-per-iter", type=int, default=1000)
-    ap.add_argument("--walkers", type=int, default=32768)
+    ap.add_argument("--walkers", type=int, default=None,
+                    help="walkers (default 32768; 65536 for the rbmpj workload)")
     ap.add_argument("--N", type=int, default=56)
     ap.add_argument("--omega", type=float, default=0.1)
     ap.add_argument("--dt", type=float, default=0.005)
@@ -429,6 +438,8 @@ def main():
     ap.add_argument("--batch-tune-steps", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
+    if args.walkers is None:
+        args.walkers = 65536 if args.workload == "rbmpj" else 32768
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this engine runs independent walkers on one device "

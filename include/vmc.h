@@ -68,7 +68,20 @@ typedef struct vmc_config {
                                density (Umrigar, Nightingale, Runge, J. Chem. Phys. 99, 2865
                                (1993)), so walkers next to a node of the Slater determinant
                                are not frozen by its divergent drift; 0 = textbook Langevin */
+    int rbm_matrix;         /* RBM-product hidden-layer contractions (pre-activation of every
+                               proposal, d ln psi/dx after every accepted move):
+                               VMC_RBM_AUTO, VMC_RBM_LANES, VMC_RBM_MFMA_F32, VMC_RBM_MFMA_BF16 */
 } vmc_config;
+
+/* rbm_matrix: AUTO = FP32 matrix cores where they apply (FP32 walk, RBM-product,
+ * >= 4 lanes per walker), lanes otherwise; LANES = FMA chains on the walker's
+ * lanes; MFMA_F32 = 16x16x4 FP32 matrix-core tiles; MFMA_BF16 = 16x16x32 BF16
+ * tiles (inputs rounded to BF16, FP32 accumulation).  The MFMA values are
+ * rejected where they do not apply. */
+#define VMC_RBM_AUTO 0
+#define VMC_RBM_LANES 1
+#define VMC_RBM_MFMA_F32 2
+#define VMC_RBM_MFMA_BF16 3
 
 /* Sums over all walkers of one sampling iteration (ref src/sampler.cpp:52-90,
  * the quantities Sampler::computeTotals reduces, ref src/sampler.cpp:111-187). */

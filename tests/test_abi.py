@@ -33,6 +33,14 @@ def test_parameter_shapes_match_reference():
     (dict(num_particles=6, num_walkers=0), "num_walkers"),
     (dict(num_particles=6, omega=-1.0), "omega"),
     (dict(num_particles=6, sampling=7), "sampling"),
+    (dict(num_particles=6, rbm_matrix=9), "rbm_matrix"),
+    # matrix-core RBM contractions: FP32 walks with an RBM-product element, K >= 4
+    (dict(num_particles=12, kinds=[V.SLATER, V.RBM_GAUSSIAN, V.RBM_PRODUCT, V.PADE],
+          num_hidden=24, precision=64, rbm_matrix=V.RBM_MFMA_F32), "matrix-core"),
+    (dict(num_particles=6, kinds=[V.SLATER, V.GAUSSIAN, V.PADE], precision=32,
+          rbm_matrix=V.RBM_MFMA_BF16), "matrix-core"),
+    (dict(num_particles=2, kinds=[V.RBM_GAUSSIAN, V.RBM_PRODUCT], num_hidden=2, precision=32,
+          rbm_matrix=V.RBM_MFMA_F32), "matrix-core"),
 ])
 def test_invalid_configs_are_rejected(kw, msg):
     args = dict(num_particles=6)

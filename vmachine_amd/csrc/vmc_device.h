@@ -55,6 +55,7 @@ struct WalkParams {
     double isig2_p;     // 1 / sigma_p^2, sigma_p = 10 sigma (ref rbmproduct.cpp:15)
     int q13_psi_ratio;  // reference RBM-Gaussian ratio exp(d/(2 sigma^2))
     int q6_w_unscaled;  // reference dlnpsi/dW without 1/sigma_p^2
+    int rbm_mf;         // RBM-product contractions: 0 lanes, 1 FP32 matrix cores, 2 BF16 matrix cores
     const double* rbm_a;    // [ND]
     const double* rbm_b;    // [H]
     const double* rbm_W;    // [ND][H], W[k][h] at k*H + h

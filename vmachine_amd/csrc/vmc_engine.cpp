@@ -105,6 +105,7 @@ struct vmc_engine {
     vmc_config cfg;
     int np = 0;             // particles per ownership slot (Slater orbitals per spin)
     vmc::KernelShape shape{};
+    int rbm_mf = 0;                // RBM-product contractions: 0 lanes, 1 FP32 MFMA, 2 BF16 MFMA
     int nd = 0;
     int maxpar = 0;
     int row_gauss = -1, row_pade = -1, row_rbmg = -1, row_rbmp = -1;
@@ -268,12 +269,6 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
             if (v >= half) { np = v; break; }
     }
 
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-        return fail(VMC_E_DEVICE, "no HIP device visible");
-
-    vmc_engine* e = new vmc_engine();
-    e->cfg = c;
     int K = 0;
     for (size_t i = 0; i < sizeof(kNP) / sizeof(kNP[0]); i++)
         if (kNP[i] == np) K = kLanes[i];
@@ -285,8 +280,23 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
         for (const auto& sh : kRbmShapes) found |= (sh[0] == np && sh[1] == K);
         if (!found) return fail(VMC_E_INVALID, "no RBM kernel for this particle/hidden-unit count");
     }
+    if (c.rbm_matrix < VMC_RBM_AUTO || c.rbm_matrix > VMC_RBM_MFMA_BF16)
+        return fail(VMC_E_INVALID, "rbm_matrix must be one of the VMC_RBM_* values");
+    const bool mf_ok = has(&c, VMC_RBM_PRODUCT) && c.precision == 32 && K >= 4;
+    if (c.rbm_matrix >= VMC_RBM_MFMA_F32 && !mf_ok)
+        return fail(VMC_E_INVALID, "matrix-core RBM contractions need an FP32 walk with an "
+                                   "RBM-product element and >= 4 lanes per walker");
+    int rbm_mf = 0;
+    if (mf_ok && c.rbm_matrix != VMC_RBM_LANES) rbm_mf = c.rbm_matrix == VMC_RBM_MFMA_BF16 ? 2 : 1;
 
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(VMC_E_DEVICE, "no HIP device visible");
+
+    vmc_engine* e = new vmc_engine();
+    e->cfg = c;
     e->np = np;
+    e->rbm_mf = rbm_mf;
     e->shape = vmc::KernelShape{np, K, rbm};
     e->nd = 2 * c.num_particles;
     e->maxpar = vmc_max_parameters(&c);
@@ -462,6 +472,7 @@ static vmc::WalkParams base_params(const vmc_engine* e)
     P.isig2_p = 1.0 / ((10.0 * c.sigma) * (10.0 * c.sigma));  // ref rbmproduct.cpp:15-17
     P.q13_psi_ratio = (c.quirks & VMC_Q13_RBMG_PSI_RATIO) ? 1 : 0;
     P.q6_w_unscaled = (c.quirks & VMC_Q6_RBM_W_UNSCALED) ? 1 : 0;
+    P.rbm_mf = e->rbm_mf;
     P.rbm_a = e->d_rbm_a;
     P.rbm_b = e->d_rbm_b;
     P.rbm_W = e->d_rbm_W;

@@ -258,6 +258,39 @@ __device__ __forceinline__ void sfor(F&& f)
 }
 
 // ------------------------------------------------------------------------
+// 16 x 16 matrix-core tiles (CDNA3/4 lane layout): element e of lane t holds
+// contraction index mf_index(q, e), q = t / 16, of row / column t % 16 of the
+// A / B operand; result element r of lane t is row 4q + r, column t % 16.
+// MODE 1: FP32 inputs, eight V_MFMA_F32_16X16X4_F32 (index 4e + q);
+// MODE 2: BF16 inputs (round to nearest even), one V_MFMA_F32_16X16X32_BF16
+// (index 8q + e).  Both accumulate in FP32.
+// ------------------------------------------------------------------------
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mf_b8 __attribute__((ext_vector_type(8)));
+
+template <int MODE>
+__device__ __forceinline__ int mf_index(int q, int e) { return MODE == 1 ? 4 * e + q : 8 * q + e; }
+
+template <int MODE>
+__device__ __forceinline__ mf_f4 mf_tile(const float (&a)[8], const float (&b)[8])
+{
+    mf_f4 c = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (MODE == 1) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[e], c, 0, 0, 0);
+    } else {
+        mf_b8 av, bv;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            av[e] = (__bf16)a[e];
+            bv[e] = (__bf16)b[e];
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+    }
+    return c;
+}
+
+// ------------------------------------------------------------------------
 // walker state
 // ------------------------------------------------------------------------
 template <typename R, int NP, int K, bool RB>
@@ -299,6 +332,9 @@ struct Walker {
     R* xs_cur;                  // this walker's sorted coordinates
     R* xs_prop;                 // the proposal's sorted coordinates (scratch)
     R* ln;                      // this walker's n_h, h = 0..H-1
+    R* mf_x;                    // [G][32] matrix-core results of the wave's walkers
+    R* mf_xs;                   // every walker's sorted vectors, [G][2][ND]
+    R* mf_n;                    // every walker's n, [G][H]
 
     __device__ __forceinline__ bool hvalid(const WalkParams& P, int j) const { return l + K * j < P.H; }
     __device__ __forceinline__ R isp2(const WalkParams& P) const { return R(P.isig2_p); }
@@ -552,16 +588,107 @@ struct Walker {
 
     // ---- RBM elements (ref src/WaveFunctions/rbmgaussian.cpp, rbmproduct.cpp) ----
 
+    // RBM-product contractions on the matrix cores (FP32 walks, K >= 4, ND and
+    // H <= 32).  The wave's G <= 16 walkers are the columns of 16 x 16 tiles;
+    // hidden units (pre-activation) or coordinates (gradient) are the rows.
+    __device__ __forceinline__ bool mf_on(const WalkParams& P) const
+    {
+        if constexpr (sizeof(R) == 4 && K >= 4) return P.rbm_mf != 0;
+        return false;
+    }
+
+    // the pre-activation sums sum_k W[k][h] x'_k of the proposal's sorted
+    // vector (xs_prop, all walkers staged) for the lane's hidden units
+    template <int MODE>
+    __device__ __forceinline__ void rbm_mf_pre(const WalkParams& P, R (&acc)[HJ]) const
+    {
+        if constexpr (sizeof(R) == 4 && K >= 4) {
+            const int t = threadIdx.x & 63, c = t & 15, q = t >> 4;
+            const int wv = c < G ? c : G - 1;
+            const R* xp = mf_xs + wv * 2 * P.ND + P.ND;
+            float b[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k = mf_index<MODE>(q, e);
+                const float v = xp[k < P.ND ? k : P.ND - 1];
+                b[e] = (c < G && k < P.ND) ? v : 0.f;
+            }
+#pragma unroll
+            for (int th = 0; th < 2; th++)
+                if (16 * th < P.H) {
+                    const int h = 16 * th + c;   // < 4K: H <= 4K
+                    const int ho = 4 * (h % K) + h / K;
+                    float a[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) a[e] = lW[mf_index<MODE>(q, e) * 4 * K + ho];
+                    const mf_f4 d = mf_tile<MODE>(a, b);
+                    if (c < G) *reinterpret_cast<mf_f4*>(mf_x + c * 32 + 16 * th + 4 * q) = d;
+                }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < HJ; j++) {
+                const int h = l + K * j;
+                acc[j] = hvalid(P, j) ? mf_x[g * 32 + (h < 32 ? h : 31)] : R(0);
+            }
+            __syncthreads();
+        }
+    }
+
+    // the gradient sums sum_h W[k][h] n_h of the staged n (all walkers) for
+    // the lane's owned coordinates
+    template <int MODE>
+    __device__ __forceinline__ void rbm_mf_grad(const WalkParams& P)
+    {
+        if constexpr (sizeof(R) == 4 && K >= 4) {
+            const int t = threadIdx.x & 63, c = t & 15, q = t >> 4;
+            const int wv = c < G ? c : G - 1;
+            const R* np_ = mf_n + wv * P.H;
+            float b[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int h = mf_index<MODE>(q, e);
+                const float v = np_[h < P.H ? h : P.H - 1];
+                b[e] = (c < G && h < P.H) ? v : 0.f;
+            }
+#pragma unroll
+            for (int tk = 0; tk < 2; tk++)
+                if (16 * tk < P.ND) {
+                    const int k = 16 * tk + c;   // rows up to 32 exist (zero from ND on)
+                    float a[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const int h = mf_index<MODE>(q, e);
+                        const int hc = h < 4 * K ? h : 4 * K - 1;
+                        const float v = lW[k * 4 * K + 4 * (hc % K) + hc / K];
+                        a[e] = h < 4 * K ? v : 0.f;
+                    }
+                    const mf_f4 d = mf_tile<MODE>(a, b);
+                    if (c < G) *reinterpret_cast<mf_f4*>(mf_x + c * 32 + 16 * tk + 4 * q) = d;
+                }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+#pragma unroll
+                for (int d = 0; d < 2; d++)
+                    {
+                    const int k = 2 * (s * NP + l) + d;
+                    gR[s][d] = valid[s] ? isp2(P) * mf_x[g * 32 + (k < 32 ? k : 31)] : R(0);
+                }
+            __syncthreads();
+        }
+    }
+
     // parameters of the lane's hidden units and owned coordinates; W into LDS
     // (block-wide, before any other RBM call)
     __device__ __forceinline__ void rbm_setup(const WalkParams& P)
     {
         const int ND = P.ND, H = P.H, K4 = 4 * K;
         if (P.has_rbmp) {
-            for (int i = threadIdx.x; i < ND * K4; i += 64) {
+            const int rows = ND > 32 ? ND : 32;   // zero rows pad the matrix-core tiles
+            for (int i = threadIdx.x; i < rows * K4; i += 64) {
                 const int k = i / K4, r = i - k * K4;
                 const int h = (r >> 2) + K * (r & 3);
-                lW[i] = h < H ? R(P.rbm_W[k * H + h]) : R(0);
+                lW[i] = (h < H && k < ND) ? R(P.rbm_W[k * H + h]) : R(0);
             }
 #pragma unroll
             for (int j = 0; j < HJ; j++) {
@@ -609,18 +736,25 @@ struct Walker {
     __device__ __forceinline__ void rbm_refresh_grad(const WalkParams& P)
     {
         const int K4 = 4 * K;
+        if (mf_on(P)) {
+            if (P.rbm_mf == 2)
+                rbm_mf_grad<2>(P);
+            else
+                rbm_mf_grad<1>(P);
+        } else {
 #pragma unroll
-        for (int s = 0; s < 2; s++)
+            for (int s = 0; s < 2; s++)
 #pragma unroll
-            for (int d = 0; d < 2; d++) {
-                R sum = R(0);
-                if (valid[s]) {
-                    const int k = 2 * (s * NP + l) + d;
-                    for (int h = 0; h < P.H; h++)
-                        sum += lW[k * K4 + 4 * (h & (K - 1)) + h / K] * ln[h];
+                for (int d = 0; d < 2; d++) {
+                    R sum = R(0);
+                    if (valid[s]) {
+                        const int k = 2 * (s * NP + l) + d;
+                        for (int h = 0; h < P.H; h++)
+                            sum += lW[k * K4 + 4 * (h & (K - 1)) + h / K] * ln[h];
+                    }
+                    gR[s][d] = isp2(P) * sum;
                 }
-                gR[s][d] = isp2(P) * sum;
-            }
+        }
         double t = 0;
 #pragma unroll
         for (int j = 0; j < HJ; j++) t += double(rc[j]) * double(rn[j]) * double(rp[j]);
@@ -697,7 +831,9 @@ struct Walker {
                 rbm_stage_n(P);
             }
             __syncthreads();
-            if (accepted) rbm_refresh_grad(P);
+            // the matrix cores take the whole wave: walkers that kept their
+            // state get their previous values again (the same n, the same sums)
+            if (mf_on(P) || accepted) rbm_refresh_grad(P);
         }
         if (accepted) local_energy(P);
     }
@@ -902,15 +1038,28 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
             R acc[WK::HJ];
 #pragma unroll
             for (int j = 0; j < WK::HJ; j++) acc[j] = R(0);
-            for (int i = 0; i < P.ND; i++) {
+            // entry i of the proposal's sorted vector
+            auto sorted_at = [&](int i) {
                 const int ii = i - (i > rq ? 1 : 0);
                 int src = ii + (ii >= ro ? 1 : 0);
                 src = src < P.ND ? src : P.ND - 1;
                 const R xs = w.xs_cur[src];
-                const R xi = (i == rq) ? xnd : xs;
-                if ((i & (K - 1)) == w.l) w.xs_prop[i] = xi;
+                return (i == rq) ? xnd : xs;
+            };
+            if (w.mf_on(P)) {
+                for (int i = w.l; i < P.ND; i += K) w.xs_prop[i] = sorted_at(i);
+                __syncthreads();
+                if (P.rbm_mf == 2)
+                    w.template rbm_mf_pre<2>(P, acc);
+                else
+                    w.template rbm_mf_pre<1>(P, acc);
+            } else {
+                for (int i = 0; i < P.ND; i++) {
+                    const R xi = sorted_at(i);
+                    if ((i & (K - 1)) == w.l) w.xs_prop[i] = xi;
 #pragma unroll
-                for (int j = 0; j < WK::HJ; j++) acc[j] += w.lW[i * K4 + 4 * w.l + j] * xi;
+                    for (int j = 0; j < WK::HJ; j++) acc[j] += w.lW[i * K4 + 4 * w.l + j] * xi;
+                }
             }
             R q = R(1), gt = R(0);
 #pragma unroll
@@ -1422,7 +1571,8 @@ struct RbmLds {
     double* accW;    // [2][ND*H] sum over the wave's walkers of xs_k n_h, and of E xs_k n_h
     double* accAB;   // [2][ND + H] the a and b parts of the same sums
     double* E;       // [G] staged local energies
-    R* W;            // [ND][4K]
+    R* W;            // [max(ND, 32)][4K], rows from ND on zero
+    R* X;            // [G][32] matrix-core results, one row per walker
     R* xs;           // [G][2][ND] sorted coordinates: current, proposal
     R* n;            // [G][H]
     __device__ __forceinline__ void carve(double* base, int ND, int H)
@@ -1431,8 +1581,9 @@ struct RbmLds {
         accW = base;
         accAB = accW + 2 * ND * H;
         E = accAB + 2 * (ND + H);
-        W = reinterpret_cast<R*>(E + G);
-        xs = W + ND * 4 * K;
+        W = reinterpret_cast<R*>(E + G);   // 16-byte aligned: ND and G are even
+        X = W + (ND > 32 ? ND : 32) * 4 * K;
+        xs = X + G * 32;
         n = xs + G * 2 * ND;
     }
 };
@@ -1441,13 +1592,17 @@ inline size_t rbm_lds_bytes(int K, int ND, int H, size_t rsize)
 {
     const int G = 64 / K;
     return sizeof(double) * (2 * ND * H + 2 * (ND + H) + G) +
-           rsize * ((size_t)ND * 4 * K + (size_t)G * 2 * ND + (size_t)G * H);
+           rsize * ((size_t)(ND > 32 ? ND : 32) * 4 * K + (size_t)G * 32 + (size_t)G * 2 * ND +
+                    (size_t)G * H);
 }
 
 template <typename R, int NP, int K, bool RB>
 __device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& L, const WalkParams& P)
 {
     w.lW = L.W;
+    w.mf_x = L.X;
+    w.mf_xs = L.xs;
+    w.mf_n = L.n;
     w.xs_cur = L.xs + (size_t)w.g * 2 * P.ND;
     w.xs_prop = w.xs_cur + P.ND;
     w.ln = L.n + (size_t)w.g * P.H;

@@ -16,6 +16,8 @@ HEADER = os.path.join(ROOT, "include", "vmc.h")
 
 GAUSSIAN, SLATER, PADE, RBM_GAUSSIAN, RBM_PRODUCT = 1, 2, 3, 4, 5
 DENSITY_RADIAL, DENSITY_SPATIAL, DENSITY_TWO_BODY = 1, 2, 4
+# RBM-product contractions (vmc_config.rbm_matrix)
+RBM_AUTO, RBM_LANES, RBM_MFMA_F32, RBM_MFMA_BF16 = 0, 1, 2, 3
 BRUTE_FORCE, IMPORTANCE, IMPORTANCE_EXACT, LEARNED = 0, 1, 2, 3
 LP_NPARAM = 5636
 Q2_PADE_RATIO_UPPER, Q3_PADE_GAMMA_HALF = 1 << 2, 1 << 3
@@ -36,6 +38,7 @@ class VmcConfig(ctypes.Structure):
         ("num_elements", _I), ("kinds", _I * 6), ("num_hidden", _I), ("sigma", _D),
         ("sorting", _I), ("sampling", _I), ("step_length", _D), ("quirks", _U),
         ("precision", _I), ("refresh_interval", _I), ("num_walkers", _I), ("drift_limit", _D),
+        ("rbm_matrix", _I),
     ]
 
 
@@ -142,9 +145,10 @@ def _dp(a):
 def make_config(num_particles, omega=1.0, interaction=True, kinds=(SLATER, GAUSSIAN, PADE),
                 sampling=IMPORTANCE, step_length=0.01, quirks=QUIRKS_REFERENCE, precision=64,
                 refresh_interval=0, num_walkers=1024, num_hidden=None, sigma=None,
-                sorting=False, num_dimensions=2, drift_limit=0.0):
+                sorting=False, num_dimensions=2, drift_limit=0.0, rbm_matrix=RBM_AUTO):
     c = VmcConfig()
     c.drift_limit = drift_limit
+    c.rbm_matrix = rbm_matrix
     c.num_particles = num_particles
     c.num_dimensions = num_dimensions
     c.omega = omega
