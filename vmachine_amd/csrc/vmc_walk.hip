@@ -1913,8 +1913,12 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     }
 }
 
+// FP32 RBM instantiations: capped at 256 registers for 2 waves/SIMD (they
+// need ~330; the spilled ones are cold enough that the second wave wins,
+// 1.2x at N=12, H=24)
 template <typename R, int NP, int K, bool RB, bool LP, bool DN>
-__global__ __launch_bounds__(64) void walk_kernel(const WalkParams P)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((RB && sizeof(R) == 4) ? 2 : 1)))
+void walk_kernel(const WalkParams P)
 {
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     extern __shared__ double rbm_dyn[];
