@@ -176,7 +176,8 @@ def rbmpj_workload(args, V):
     res = {}
     forms = [("fp32", 32, V.RBM_AUTO), ("fp32_lanes", 32, V.RBM_LANES),
              ("fp32_mfma_bf16", 32, V.RBM_MFMA_BF16), ("fp64", 64, V.RBM_AUTO)]
-    for name, prec, mode in forms:
+    keep = set(str(args.rbm_forms).split(","))
+    for name, prec, mode in [f for f in forms if f[0] in keep]:
         cfg = V.make_config(N, omega=1.0, kinds=[V.SLATER, V.RBM_GAUSSIAN, V.RBM_PRODUCT, V.PADE],
                             sampling=V.IMPORTANCE, step_length=0.01, precision=prec,
                             refresh_interval=args.refresh if prec == 32 else 0,
@@ -194,9 +195,12 @@ def rbmpj_workload(args, V):
         print(f"rbmpj {name}: {ws / kern:.4g} walker-steps/s (kernel), E {E:.6f}",
               file=sys.stderr, flush=True)
         eng.close()
+    if "opt" in keep:
+        res["optimisation"] = rbmpj_optimisation(args, V, N, H, P)
     out = {"metric": "walker-steps/s, RBM x Slater x Pade-Jastrow N=12 H=24",
-           "value": res["fp32"]["walker_steps_per_s"], "unit": "walker-steps/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_step"],
+           "value": res.get("fp32", {}).get("walker_steps_per_s"), "unit": "walker-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res.get("fp32", {}).get("ms_per_step"),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 walk, f64 inverse rebuild and accumulation",
            "data": "synthetic (walkers drawn from N(0, 1/omega^2), burned in; Xavier weights)",
@@ -208,6 +212,45 @@ def rbmpj_workload(args, V):
                       "rbm_contractions": "FP32 matrix-core tiles (V_MFMA_F32_16X16X4_F32)"},
            "forms": res}
     print(json.dumps(out))
+
+
+def rbmpj_optimisation(args, V, N, H, P):
+    """ADAM (lr 0.01, 10 gradient batches) iterations of --steps-per-iter steps,
+    FP32 walk: on the device (vmc_optimize, one hipGraph per iteration) and as
+    the host loop the reference runs (vmc_run, host ADAM, vmc_set_parameters)."""
+    from vmachine_amd import host as HST
+    cfg = V.make_config(N, omega=1.0, kinds=[V.SLATER, V.RBM_GAUSSIAN, V.RBM_PRODUCT, V.PADE],
+                        sampling=V.IMPORTANCE, step_length=0.01, precision=32,
+                        refresh_interval=args.refresh, num_walkers=args.walkers, num_hidden=H,
+                        sigma=1.0, sorting=True)
+    S, iters = args.steps_per_iter, args.opt_iters
+    ws = args.walkers * S * iters
+    out = {}
+    with V.Engine(cfg) as eng:
+        eng.set_parameters(P)
+        eng.init_walkers(args.seed)
+        burn_in(eng, args.burnin, "rbmpj optimisation")
+        Pd, _, _ = eng.optimize(P, 1, S, 0, kind="adam", learning_rate=0.01)   # warm-up
+        t0 = time.perf_counter()
+        Pd, hist, ms = eng.optimize(Pd, iters, S, 0, kind="adam", learning_rate=0.01)
+        wall = time.perf_counter() - t0
+        out["device_loop"] = {"iterations_per_s": iters / wall, "walker_steps_per_s": ws / wall,
+                              "kernel_walker_steps_per_s": ws / (ms / 1e3),
+                              "energy_last": float(hist[-1, 0] / (args.walkers * S))}
+        opt = HST.Optimizer("adam", P.size, 0.01)
+        Ph = Pd.copy()
+        t0 = time.perf_counter()
+        for it in range(iters):
+            eng.set_parameters(Ph)
+            sums, g, ge, _ = eng.run(S, 0, it, opt.batches)
+            upd = opt.update(sums["energy"] / sums["samples"], g / sums["gradient_samples"],
+                             ge / sums["gradient_samples"])
+            Ph = Ph - upd.reshape(Ph.shape)
+        wall = time.perf_counter() - t0
+        out["host_loop"] = {"iterations_per_s": iters / wall, "walker_steps_per_s": ws / wall}
+    print(f"rbmpj optimisation: device loop {out['device_loop']['iterations_per_s']:.3g} it/s, "
+          f"host loop {out['host_loop']['iterations_per_s']:.3g} it/s", file=sys.stderr, flush=True)
+    return out
 
 
 BATCH_DOTS = [(2, 1.0), (6, 1.0), (12, 1.0), (20, 0.5), (30, 0.5), (42, 0.28), (56, 0.1)]
@@ -437,6 +480,11 @@ def main():
     ap.add_argument("--batch-lr", type=float, default=0.01)
     ap.add_argument("--batch-tune-steps", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--rbm-forms", default="fp32,fp32_lanes,fp32_mfma_bf16,fp64,opt",
+                    help="rbmpj workload: the forms to run (fp32 = matrix-core FP32 tiles; "
+                         "opt = ADAM iterations, device loop vs host loop)")
+    ap.add_argument("--opt-iters", type=int, default=5,
+                    help="rbmpj workload: timed optimisation iterations per loop form")
     args = ap.parse_args()
     if args.walkers is None:
         args.walkers = 65536 if args.workload == "rbmpj" else 32768

@@ -251,6 +251,41 @@ int vmc_lp_gradient(vmc_engine* e, const double* positions, int walkers, const i
 int vmc_lp_train(vmc_engine* e, int iterations, int walk_steps, double learning_rate,
                  double* history);
 
+/* ---- Optimisation on the device ----
+ * `iterations` optimisation iterations of the engine's system without a return
+ * to the host in between: per iteration equilibration + steps steps of every
+ * walker, the sums, the reference's energy gradient g = 2(<G E> - <E><G>) and
+ * optimiser step on every entry of the parameter matrix, and the updated
+ * parameters written where the next iteration's walk reads them.  With
+ * use_graph one iteration is captured as a hipGraph and replayed.  Replaces
+ * the iteration loop of System::runSimulation (ref src/system.cpp:81-139:
+ * runMetropolisCycles, Sampler::computeTotals, Optimization::updateParameters;
+ * ref src/Optimization/optimization.cpp:10-16, adam.cpp:12-38, sgd.cpp:15-29,
+ * gradientdescent.cpp:15-30).  The optimiser starts as
+ * Optimization::initialize leaves it (moments of one, step 0) and the
+ * gradient window of iteration i is i % batches.
+ * params [num_elements][max_parameters]: in, the starting matrix; out, the
+ * final one, which the engine keeps (as after vmc_set_parameters).
+ * history [iterations][VMC_OPT_OUT] (may be null): the iteration's 14 sums
+ * over walkers and sampled steps (E, E^2, T, Vext, Vint, T^2, Vext^2, Vint^2,
+ * accepted, G_alpha, G_gamma, G_alpha E, G_gamma E over the gradient window,
+ * non-finite E_L), then alpha and gamma after its step.  kernel_ms (may be
+ * null): device time of all iterations. */
+typedef struct vmc_optimizer_config {
+    int kind;               /* VMC_OPT_ADAM, VMC_OPT_SGD, VMC_OPT_GD */
+    double learning_rate;   /* learningRate */
+    double gamma;           /* SGD / GD momentum (the optimization key's gamma) */
+    double monotonic;       /* SGD / GD step exponent */
+    int batches;            /* gradient batches: 10 for ADAM in the reference, 1 otherwise */
+} vmc_optimizer_config;
+#define VMC_OPT_ADAM 0
+#define VMC_OPT_SGD 1
+#define VMC_OPT_GD 2
+#define VMC_OPT_OUT 16
+int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int equilibration,
+                 const vmc_optimizer_config* opt, int use_graph, double* history,
+                 double* kernel_ms);
+
 /* ---- Batches: several quantum dots walked side by side ----
  * Systems of different particle numbers and frequencies (Slater / Gaussian /
  * Pade-Jastrow elements, the same for all) are walked by one kernel each, the

@@ -35,6 +35,7 @@ UNITS = [
     ("vmc_walk_common.o", "vmc_walk.hip", ["-DVMC_WALK_PRECISION=0"]),
     ("vmc_learn.o", "vmc_learn.hip", []),
     ("vmc_batch.o", "vmc_batch.hip", []),
+    ("vmc_optim.o", "vmc_optim.hip", []),
     ("vmc_engine.o", "vmc_engine.cpp", []),
 ]
 

@@ -60,6 +60,10 @@ struct WalkParams {
     const double* rbm_b;    // [H]
     const double* rbm_W;    // [ND][H], W[k][h] at k*H + h
     double* rbm_acc;        // [blocks][rbm_row_width(ND, H)] gradient-window sums per block
+    // optimisation loop on the device (null otherwise): alpha, gamma, the
+    // random counter, gradient window and drift flag of the current iteration
+    const double* dyn_ag;   // [2] alpha, gamma
+    const struct OptState* dyn_st;
     // learned proposal (sampling == kLearned)
     const uint16_t* lp_w;   // [kLpParams] network weights, bfloat16
     double lp_sigma0;       // untrained move scale, units of 1/sqrt(omega)
@@ -145,6 +149,26 @@ struct EvalOut {
     int rows[6];       // element kind of each parameter row
     int num_elements, maxpar;
 };
+
+// ---- optimisation loop on the device (vmc_optim.hip) ----
+struct OptState {
+    uint64_t counter0;  // random counter of the iteration's first step
+    int iteration;      // iteration (gradient window = iteration % batches)
+    int batches;        // gradient batches
+    int fresh;          // 1: set the stored drift from the positions (cleared after an iteration)
+    int step;           // optimiser steps taken
+};
+
+// where the sums of each parameter of the row-major elements x maxParams
+// matrix are, and the optimiser's constants
+struct OptLayout {
+    int rows, mp, nd, H;
+    int row_gauss, row_pade, row_rbmg, row_rbmp;
+    int kind;           // 0 ADAM, 1 SGD / gradient descent (momentum form)
+    double lr, gamma, monotonic;
+    double M, Sw;       // samples and gradient-window samples per iteration
+};
+enum : int { kOptAlpha = kNumWalkerAcc, kOptGamma, kOptOut };
 
 // ---- batch of systems walked side by side (vmc_batch.hip) ----
 enum : int { kBatchAlpha = kNumWalkerAcc, kBatchGamma, kBatchOut };

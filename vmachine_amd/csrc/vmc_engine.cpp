@@ -6,6 +6,7 @@
 // iteration into launches of the walk kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -24,6 +25,10 @@ hipError_t launch_eval(int precision, const KernelShape& sh, const WalkParams& P
 // out: [1 + kRowChunks][width] (the sums, then partial rows)
 hipError_t launch_reduce_rows(const double* rows, int nrows, int width, double* out,
                               hipStream_t stream);
+hipError_t launch_opt_tail(const OptLayout& L, const double* acc, const double* rbm, double* theta,
+                           double* m, double* v, OptState* st, double* ag, double* rbm_a,
+                           double* rbm_b, double* rbm_W, double* hist, int it0, int steps_total,
+                           hipStream_t stream);
 hipError_t launch_lp_draws(const LpTrainParams& T, uint64_t iteration, int* electrons,
                            double* normals, hipStream_t stream);
 hipError_t launch_lp_propose(const LpTrainParams& T, hipStream_t stream);
@@ -610,6 +615,129 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
         }
     }
     return VMC_OK;
+}
+
+int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int equilibration,
+                 const vmc_optimizer_config* opt, int use_graph, double* history, double* kernel_ms)
+{
+    if (!e || !params || !opt) return fail(VMC_E_INVALID, "null argument");
+    if (!e->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
+    if (e->cfg.sampling == VMC_LEARNED && !e->lp_set)
+        return fail(VMC_E_STATE, "the learned proposal has no weights (vmc_lp_set)");
+    if (iterations < 1 || steps < 1 || equilibration < 0 || opt->batches < 1 ||
+        opt->batches > steps)
+        return fail(VMC_E_INVALID, "iterations, steps >= 1; equilibration >= 0; 1 <= batches <= steps");
+    if (opt->kind < VMC_OPT_ADAM || opt->kind > VMC_OPT_GD || !(opt->learning_rate >= 0))
+        return fail(VMC_E_INVALID, "optimizer kind VMC_OPT_ADAM/SGD/GD, learning_rate >= 0");
+    const int E = e->cfg.num_elements, mp = e->maxpar, n = E * mp;
+    if (int rc = vmc_set_parameters(e, params, E, mp)) return rc;
+    vmc::OptLayout L{};
+    L.rows = E;
+    L.mp = mp;
+    L.nd = e->nd;
+    L.H = e->row_rbmp >= 0 ? e->cfg.num_hidden : 0;
+    L.row_gauss = e->row_gauss;
+    L.row_pade = e->row_pade;
+    L.row_rbmg = e->row_rbmg;
+    L.row_rbmp = e->row_rbmp;
+    L.kind = opt->kind == VMC_OPT_ADAM ? 0 : 1;
+    L.lr = opt->learning_rate;
+    L.gamma = opt->gamma;
+    L.monotonic = opt->monotonic;
+    const int W = e->cfg.num_walkers, total = equilibration + steps;
+    L.M = double(W) * steps;
+    L.Sw = double(W) * (steps / opt->batches);   // ref sampler.cpp:104, 216-217
+
+    double *d_theta = nullptr, *d_m = nullptr, *d_v = nullptr, *d_ag = nullptr, *d_hist = nullptr;
+    vmc::OptState* d_st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto done = [&](int code) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : {(void*)d_theta, (void*)d_m, (void*)d_v, (void*)d_ag, (void*)d_hist, (void*)d_st})
+            if (p) (void)hipFree(p);
+        return code;
+    };
+    const std::vector<double> ones(std::max(n, 1), 1.0);   // ref adam.cpp:17-19, sgd.cpp:20
+    const double ag[2] = {e->alpha, e->gamma};
+    const vmc::OptState st0{e->counter, 0, opt->batches, e->init_qf ? 1 : 0, 0};
+    hipError_t err;
+    if ((err = hipMalloc(&d_theta, sizeof(double) * std::max(n, 1))) != hipSuccess ||
+        (err = hipMalloc(&d_m, sizeof(double) * std::max(n, 1))) != hipSuccess ||
+        (err = hipMalloc(&d_v, sizeof(double) * std::max(n, 1))) != hipSuccess ||
+        (err = hipMalloc(&d_ag, sizeof(ag))) != hipSuccess ||
+        (err = hipMalloc(&d_st, sizeof(st0))) != hipSuccess ||
+        (err = hipMalloc(&d_hist, sizeof(double) * (size_t)iterations * vmc::kOptOut)) != hipSuccess ||
+        (n > 0 && (err = hipMemcpy(d_theta, params, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (err = hipMemcpy(d_m, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (err = hipMemcpy(d_v, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (err = hipMemcpy(d_ag, ag, sizeof(ag), hipMemcpyHostToDevice)) != hipSuccess ||
+        (err = hipMemcpy(d_st, &st0, sizeof(st0), hipMemcpyHostToDevice)) != hipSuccess)
+        return done(fail(VMC_E_DEVICE, std::string("optimize set-up: ") + hipGetErrorString(err)));
+
+    vmc::WalkParams P = base_params(e);
+    P.steps = steps;
+    P.equil = equilibration;
+    P.pos = e->d_pos;
+    P.qf = (e->cfg.sampling == VMC_IMPORTANCE) ? e->d_qf : nullptr;
+    P.acc = e->d_acc;
+    P.rec = nullptr;
+    P.dyn_ag = d_ag;
+    P.dyn_st = d_st;
+    const int seg = (e->cfg.refresh_interval > 0) ? e->cfg.refresh_interval : total;
+    auto enqueue = [&]() -> hipError_t {
+        hipError_t r;
+        for (int t0 = 0; t0 < total; t0 += seg) {
+            P.t_begin = t0;
+            P.t_end = (t0 + seg < total) ? t0 + seg : total;
+            P.acc_add = t0 > 0 ? 1 : 0;
+            if ((r = vmc::launch_walk(e->cfg.precision, e->shape, P, e->stream)) != hipSuccess) return r;
+        }
+        if ((r = vmc::launch_reduce(e->d_acc, W, e->d_out, e->stream)) != hipSuccess) return r;
+        if (e->shape.rbm &&
+            (r = vmc::launch_reduce_rows(e->d_rbm_acc, e->rbm_blocks, e->rbm_width, e->d_rbm_out,
+                                         e->stream)) != hipSuccess)
+            return r;
+        return vmc::launch_opt_tail(L, e->d_out, e->d_rbm_out, d_theta, d_m, d_v, d_st, d_ag,
+                                    e->d_rbm_a, e->d_rbm_b, e->d_rbm_W, d_hist, 0, total, e->stream);
+    };
+    if (use_graph) {
+        if ((err = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return done(fail(VMC_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(err)));
+        const hipError_t e1 = enqueue();
+        const hipError_t e2 = hipStreamEndCapture(e->stream, &graph);
+        if (e1 != hipSuccess || e2 != hipSuccess)
+            return done(fail(VMC_E_DEVICE, std::string("graph capture: ") +
+                                               hipGetErrorString(e1 != hipSuccess ? e1 : e2)));
+        if ((err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+            return done(fail(VMC_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(err)));
+    }
+    if ((err = hipEventRecord(e->ev0, e->stream)) != hipSuccess)
+        return done(fail(VMC_E_DEVICE, std::string("optimize: ") + hipGetErrorString(err)));
+    for (int it = 0; it < iterations; it++) {
+        err = use_graph ? hipGraphLaunch(exec, e->stream) : enqueue();
+        if (err != hipSuccess) return done(fail(VMC_E_DEVICE, std::string("optimize: ") + hipGetErrorString(err)));
+    }
+    std::vector<double> hist((size_t)iterations * vmc::kOptOut);
+    double agf[2];
+    float ms = 0;
+    if ((err = hipEventRecord(e->ev1, e->stream)) != hipSuccess ||
+        (n > 0 && (err = hipMemcpyAsync(params, d_theta, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                        e->stream)) != hipSuccess) ||
+        (err = hipMemcpyAsync(agf, d_ag, sizeof(agf), hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(hist.data(), d_hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost,
+                              e->stream)) != hipSuccess ||
+        (err = hipStreamSynchronize(e->stream)) != hipSuccess ||
+        (err = hipEventElapsedTime(&ms, e->ev0, e->ev1)) != hipSuccess)
+        return done(fail(VMC_E_DEVICE, std::string("optimize: ") + hipGetErrorString(err)));
+    e->alpha = agf[0];
+    e->gamma = agf[1];
+    e->counter += (uint64_t)iterations * total;
+    e->init_qf = false;
+    if (history) std::copy(hist.begin(), hist.end(), history);
+    if (kernel_ms) *kernel_ms = ms;
+    return done(VMC_OK);
 }
 
 int vmc_run(vmc_engine* e, int steps, int equilibration, int iteration, int batches,

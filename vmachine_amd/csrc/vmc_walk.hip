@@ -1922,8 +1922,19 @@ void walk_kernel(const WalkParams P)
 {
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     extern __shared__ double rbm_dyn[];
-    walk_block<R, NP, K, RB, LP, DN>(P, walk_dyn(P), blockIdx.x, lds, lds + (64 / K) * NP * NP,
-                                     rbm_dyn);
+    WalkDyn D = walk_dyn(P);
+    if (P.dyn_st) {   // an optimisation loop on the device: this iteration's values
+        const OptState& st = *P.dyn_st;
+        D.alpha = P.dyn_ag[0];
+        D.gamma = P.dyn_ag[1];
+        D.counter0 = st.counter0;
+        const int S = P.steps / st.batches;   // ref sampler.cpp:104
+        const int kb = st.iteration % st.batches;
+        D.window_lo = S * kb;
+        D.window_hi = S * (kb + 1);
+        D.init_qf = st.fresh;
+    }
+    walk_block<R, NP, K, RB, LP, DN>(P, D, blockIdx.x, lds, lds + (64 / K) * NP * NP, rbm_dyn);
 }
 
 // ------------------------------------------------------------------------

@@ -60,6 +60,15 @@ class VmcBatchSystem(ctypes.Structure):
                 ("step_length", _D), ("alpha", _D), ("gamma", _D)]
 
 
+class VmcOptimizerConfig(ctypes.Structure):
+    _fields_ = [("kind", _I), ("learning_rate", _D), ("gamma", _D), ("monotonic", _D),
+                ("batches", _I)]
+
+
+OPT_ADAM, OPT_SGD, OPT_GD = 0, 1, 2
+OPT_OUT = 16   # per iteration: the 14 walk sums (kernel order), alpha, gamma after the step
+
+
 class VmcBatchConfig(ctypes.Structure):
     _fields_ = [("num_systems", _I), ("systems", ctypes.POINTER(VmcBatchSystem)),
                 ("num_elements", _I), ("kinds", _I * 6), ("interaction", _I), ("sampling", _I),
@@ -123,6 +132,8 @@ def lib():
         L.vmc_set_density.argtypes = [ctypes.c_void_p, _I, _I, _D]
         _I64P = ctypes.POINTER(ctypes.c_int64)
         L.vmc_get_density.argtypes = [ctypes.c_void_p, _I64P, _I64P, _I64P]
+        L.vmc_optimize.argtypes = [ctypes.c_void_p, _DP, _I, _I, _I,
+                                   ctypes.POINTER(VmcOptimizerConfig), _I, _DP, _DP]
         L.vmc_batch_create.argtypes = [ctypes.POINTER(VmcBatchConfig),
                                        ctypes.POINTER(ctypes.c_void_p)]
         L.vmc_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -252,6 +263,22 @@ class Engine:
         s, c = ctypes.c_uint64(), ctypes.c_uint64()
         self._check(self.L.vmc_get_random_state(self.h, ctypes.byref(s), ctypes.byref(c)))
         return s.value, c.value
+
+    def optimize(self, params, iterations, steps, equilibration=0, kind="adam",
+                 learning_rate=0.1, gamma=0.0, monotonic=0.0, batches=None, use_graph=True):
+        """Optimisation iterations on the device (vmc_optimize).  Returns (final
+        parameter matrix, history [iterations][OPT_OUT], device milliseconds);
+        batches defaults to the reference's (10 for adam, 1 otherwise)."""
+        k = {"adam": OPT_ADAM, "sgd": OPT_SGD, "gd": OPT_GD}[kind]
+        oc = VmcOptimizerConfig(k, learning_rate, gamma, monotonic,
+                                batches if batches else (10 if k == OPT_ADAM else 1))
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(self.E, self.maxpar).copy()
+        hist = np.zeros((iterations, OPT_OUT))
+        ms = ctypes.c_double(0)
+        self._check(self.L.vmc_optimize(self.h, _dp(p), iterations, steps, equilibration,
+                                        ctypes.byref(oc), int(bool(use_graph)), _dp(hist),
+                                        ctypes.byref(ms)))
+        return p, hist, ms.value
 
     def run(self, steps, equilibration=0, iteration=0, batches=1, record_walkers=0):
         sums = VmcSums()
