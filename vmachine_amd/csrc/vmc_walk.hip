@@ -1276,6 +1276,19 @@ inline size_t lp_lds_bytes(int K, size_t rsize)
     return rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLpParams;
 }
 
+template <int NP, int K>
+constexpr int walk_static_lds_doubles();
+
+// the learned proposal's LDS fits in the Slater rebuild's static LDS, which is
+// free once the rebuild is done (then a launch needs no dynamic LDS: 14.8 KB
+// instead of 26 KB per wave at N=56, 10 waves per CU instead of 6)
+template <typename R, int NP, int K>
+constexpr bool lp_fits_static()
+{
+    return walk_static_lds_doubles<NP, K>() * sizeof(double) >=
+           sizeof(R) * (64 / K) * kLpHid + sizeof(uint16_t) * kLpParams;
+}
+
 // minimum over the K lanes of a group of 32-bit keys (same pairing as gsum)
 template <int K>
 __device__ __forceinline__ uint32_t gmin32(uint32_t v)
@@ -1696,7 +1709,10 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     RbmLds<R, K> L;
     LpLds<R, K> LL;
     walker_load(w, P, P.pos + (size_t)walker * nd, P.qf ? P.qf + (size_t)walker * nd : nullptr);
-    if constexpr (LP) {
+    // the network's weights and activations: in the rebuild's LDS once the
+    // rebuild is done where they fit (large NP), in the dynamic area otherwise
+    constexpr bool lp_static = lp_fits_static<R, NP, K>();
+    if constexpr (LP && !lp_static) {
         lp_carve(LL, rbm_dyn);
         for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
         __syncthreads();
@@ -1713,6 +1729,12 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     if (P.has_slater) w.rebuild_slater(P, lds_T, lds_row);
     w.rebuild_pairs(P);
     w.local_energy(P);
+    if constexpr (LP && lp_static) {
+        __syncthreads();   // the rebuild's reads of lds_T are done
+        lp_carve(LL, lds_T);
+        for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
+        __syncthreads();
+    }
     if (D.init_qf && D.t_begin == 0) {
 #pragma unroll
         for (int s = 0; s < 2; s++) {
@@ -2076,7 +2098,7 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
     constexpr int G = 64 / K;
     const int blocks = (P.W + G - 1) / G;
     const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT))
-                          : (LP ? lp_lds_bytes(K, sizeof(RT)) : 0);
+                          : ((LP && !lp_fits_static<RT, NP, K>()) ? lp_lds_bytes(K, sizeof(RT)) : 0);
     hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN>), dim3(blocks), dim3(64), lds, stream, P);
     return hipGetLastError();
 }
