@@ -1588,25 +1588,46 @@ struct RbmLds {
     R* X;            // [G][32] matrix-core results, one row per walker
     R* xs;           // [G][2][ND] sorted coordinates: current, proposal
     R* n;            // [G][H]
-    __device__ __forceinline__ void carve(double* base, int ND, int H)
+    // W_elsewhere: W in other LDS (the rebuild's), not in the dynamic area
+    __device__ __forceinline__ void carve(double* base, int ND, int H, R* W_elsewhere = nullptr)
     {
         constexpr int G = 64 / K;
         accW = base;
         accAB = accW + 2 * ND * H;
         E = accAB + 2 * (ND + H);
-        W = reinterpret_cast<R*>(E + G);   // 16-byte aligned: ND and G are even
-        X = W + (ND > 32 ? ND : 32) * 4 * K;
+        R* p = reinterpret_cast<R*>(E + G);   // 16-byte aligned: ND and G are even
+        if (W_elsewhere) {
+            W = W_elsewhere;
+        } else {
+            W = p;
+            p += (ND > 32 ? ND : 32) * 4 * K;
+        }
+        X = p;
         xs = X + G * 32;
         n = xs + G * 2 * ND;
     }
 };
 
-inline size_t rbm_lds_bytes(int K, int ND, int H, size_t rsize)
+inline size_t rbm_lds_bytes(int K, int ND, int H, size_t rsize, bool w_elsewhere = false)
 {
     const int G = 64 / K;
+    const size_t wsz = w_elsewhere ? 0 : (size_t)(ND > 32 ? ND : 32) * 4 * K;
     return sizeof(double) * (2 * ND * H + 2 * (ND + H) + G) +
-           rsize * ((size_t)(ND > 32 ? ND : 32) * 4 * K + (size_t)G * 32 + (size_t)G * 2 * ND +
-                    (size_t)G * H);
+           rsize * (wsz + (size_t)G * 32 + (size_t)G * 2 * ND + (size_t)G * H);
+}
+
+// LDS the walk of one block needs besides the dynamic RBM / learned-proposal
+// area: the Gauss-Jordan transpose and pivot rows of the Slater rebuild
+template <int NP, int K>
+constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 5 * (64 / K) * NP; }
+
+// the walk's copy of the RBM-product W (32 rows: N D <= 32) fits in the Slater
+// rebuild's static LDS, free once the launch's rebuild is done (N=12 FP32:
+// 17.5 KB instead of 21.6 KB per wave, 9 waves per CU instead of 7)
+template <typename R, int NP, int K>
+constexpr bool rbm_w_fits_static()
+{
+    return walk_static_lds_doubles<NP, K>() * sizeof(double) >= 32 * 4 * K * sizeof(R);
 }
 
 template <typename R, int NP, int K, bool RB>
@@ -1624,16 +1645,12 @@ __device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& 
 // ------------------------------------------------------------------------
 // the walk kernel: equil + steps Metropolis steps per walker
 // ------------------------------------------------------------------------
-// LDS the walk of one block needs besides the dynamic RBM / learned-proposal
-// area: the Gauss-Jordan transpose and pivot rows of the Slater rebuild
+// the per-launch values of a single-system launch
 __device__ __forceinline__ WalkDyn walk_dyn(const WalkParams& P)
 {
     return WalkDyn{P.alpha,   P.gamma,     P.counter0,  P.t_begin, P.t_end,
                    P.acc_add, P.window_lo, P.window_hi, P.init_qf};
 }
-
-template <int NP, int K>
-constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 5 * (64 / K) * NP; }
 
 // electron densities of the walker's current state (ref src/sampler.cpp:639-692;
 // r_i = |x_i|, ref Metropolis::calculateRadialVectorElement, metropolis.cpp:31-39):
@@ -1717,16 +1734,23 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
         for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
         __syncthreads();
     }
-    if constexpr (RB) {
-        L.carve(rbm_dyn, P.ND, P.H);
+    // the RBM state; W in the rebuild's LDS after the rebuild where it fits
+    constexpr bool w_static = rbm_w_fits_static<R, NP, K>();
+    auto rbm_start = [&](R* W_elsewhere) {
+        L.carve(rbm_dyn, P.ND, P.H, W_elsewhere);
         rbm_bind(w, L, P);
         for (int i = lane; i < 2 * P.ND * P.H; i += 64) L.accW[i] = 0.0;
         for (int i = lane; i < 2 * (P.ND + P.H); i += 64) L.accAB[i] = 0.0;
         w.rbm_setup(P);
         __syncthreads();
         w.rbm_init(P);
-    }
+    };
+    if constexpr (RB && !w_static) rbm_start(nullptr);
     if (P.has_slater) w.rebuild_slater(P, lds_T, lds_row);
+    if constexpr (RB && w_static) {
+        __syncthreads();   // the rebuild's reads of lds_T are done
+        rbm_start(reinterpret_cast<R*>(lds_T));
+    }
     w.rebuild_pairs(P);
     w.local_energy(P);
     if constexpr (LP && lp_static) {
@@ -2097,7 +2121,7 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
 {
     constexpr int G = 64 / K;
     const int blocks = (P.W + G - 1) / G;
-    const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT))
+    const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT), rbm_w_fits_static<RT, NP, K>())
                           : ((LP && !lp_fits_static<RT, NP, K>()) ? lp_lds_bytes(K, sizeof(RT)) : 0);
     hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN>), dim3(blocks), dim3(64), lds, stream, P);
     return hipGetLastError();
