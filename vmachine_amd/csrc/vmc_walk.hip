@@ -2161,17 +2161,20 @@ hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, bool lp, const WalkP
 {
     if (lp) {
         if (rbm) return hipErrorInvalidValue;
-#define Y(a, b) \
-    if (np == a && K == b) return launch_walk_t<a, b, false, true, true>(P, stream);
+#define Y(a, b)                                                                       \
+    if (np == a && K == b)                                                            \
+        return P.dens_flags ? launch_walk_t<a, b, false, true, true>(P, stream)       \
+                            : launch_walk_t<a, b, false, true, false>(P, stream);
         VMC_LP_SHAPES(Y)
 #undef Y
         return hipErrorInvalidValue;
     }
-    // the plain shapes have an instantiation without the density code
-#define X(a, b, c)                                                                  \
-    if (np == a && K == b && rbm == c)                                              \
-        return (c || P.dens_flags) ? launch_walk_t<a, b, c, false, true>(P, stream) \
-                                   : launch_walk_t<a, b, c, false, c>(P, stream);
+    // every shape has an instantiation without the density code (its registers
+    // and scheduling stay out of the walks that count no densities)
+#define X(a, b, c)                                                              \
+    if (np == a && K == b && rbm == c)                                          \
+        return P.dens_flags ? launch_walk_t<a, b, c, false, true>(P, stream)    \
+                            : launch_walk_t<a, b, c, false, false>(P, stream);
     VMC_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
