@@ -418,13 +418,33 @@ struct Walker {
                 const uint32_t key = used[b] ? 0u : ((hiw & ~31u) | (uint32_t)l);
                 idx[b] = (int)(gmax32<K>(key) & 31u);
             }
+            if constexpr (NBK == 2) {
+                // both blocks' pivot rows leave in one set of (masked) LDS
+                // writes: each pivot lane writes its block's row; a lane that
+                // pivots both blocks writes the second row separately
+                double* row0 = lds_row + (((c & 1) * 2 + b0) * G + g) * NP;
+                double* row1 = lds_row + (((c & 1) * 2 + b0 + 1) * G + g) * NP;
+                const bool w0 = l == idx[0], w1 = l == idx[1];
+                if (w0 || w1) {
+                    double* dst = w0 ? row0 : row1;
 #pragma unroll
-            for (int b = 0; b < NBK; b++) {
-                double* row = lds_row + (((c & 1) * 2 + b0 + b) * G + g) * NP;
-                if (l == idx[b]) {
+                    for (int j = 0; j < NP; j++) dst[j] = w0 ? a[0][j] : a[1][j];
+                    piv[((b0 + (w0 ? 0 : 1)) * G + g) * NP + c] = w0 ? idx[0] : idx[1];
+                }
+                if (w0 && w1) {
 #pragma unroll
-                    for (int j = 0; j < NP; j++) row[j] = a[b][j];
-                    piv[((b0 + b) * G + g) * NP + c] = idx[b];
+                    for (int j = 0; j < NP; j++) row1[j] = a[1][j];
+                    piv[((b0 + 1) * G + g) * NP + c] = idx[1];
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < NBK; b++) {
+                    double* row = lds_row + (((c & 1) * 2 + b0 + b) * G + g) * NP;
+                    if (l == idx[b]) {
+#pragma unroll
+                        for (int j = 0; j < NP; j++) row[j] = a[b][j];
+                        piv[((b0 + b) * G + g) * NP + c] = idx[b];
+                    }
                 }
             }
             __syncthreads();
