@@ -1281,19 +1281,39 @@ template <typename R, int K>
 struct LpLds {
     uint16_t* W;   // [kLpParams]
     R* h;          // [G][kLpHid] first-layer activations of each walker
+    float* xa;     // [G][kLpHid] matrix-core pre-activations (FP32 walks, K >= 16)
+    uint32_t* fs;  // [G][16] features as bf16 pairs, zero from 18 on (FP32 walks, K >= 16)
     __device__ __forceinline__ R w(int i) const { return R(bf16_to_float(W[i])); }
 };
+
+// FP32 walks with K >= 16 (G <= 4 walkers per wave) run the network's hidden
+// layers on BF16 matrix-core tiles
+constexpr bool lp_mf(size_t rsize, int K) { return rsize == 4 && K >= 16; }
+
+// bytes of the network's LDS (weights, activations, matrix-core staging)
+constexpr size_t lp_lds_bytes_c(int K, size_t rsize)
+{
+    const size_t base = rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLpParams;
+    return lp_mf(rsize, K) ? ((base + 15) & ~size_t(15)) + sizeof(float) * (64 / K) * kLpHid +
+                                 sizeof(uint32_t) * (64 / K) * 16
+                           : base;
+}
+inline size_t lp_lds_bytes(int K, size_t rsize) { return lp_lds_bytes_c(K, rsize); }
 
 template <typename R, int K>
 __device__ __forceinline__ void lp_carve(LpLds<R, K>& L, double* base)
 {
+    constexpr int G = 64 / K;
     L.h = reinterpret_cast<R*>(base);
-    L.W = reinterpret_cast<uint16_t*>(L.h + (64 / K) * kLpHid);
-}
-
-inline size_t lp_lds_bytes(int K, size_t rsize)
-{
-    return rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLpParams;
+    L.W = reinterpret_cast<uint16_t*>(L.h + G * kLpHid);
+    if constexpr (lp_mf(sizeof(R), K)) {
+        const size_t off = ((sizeof(R) * G * kLpHid + sizeof(uint16_t) * kLpParams) + 15) & ~size_t(15);
+        L.xa = reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off);
+        L.fs = reinterpret_cast<uint32_t*>(L.xa + G * kLpHid);
+    } else {
+        L.xa = nullptr;
+        L.fs = nullptr;
+    }
 }
 
 template <int NP, int K>
@@ -1305,8 +1325,7 @@ constexpr int walk_static_lds_doubles();
 template <typename R, int NP, int K>
 constexpr bool lp_fits_static()
 {
-    return walk_static_lds_doubles<NP, K>() * sizeof(double) >=
-           sizeof(R) * (64 / K) * kLpHid + sizeof(uint16_t) * kLpParams;
+    return walk_static_lds_doubles<NP, K>() * sizeof(double) >= lp_lds_bytes_c(K, sizeof(R));
 }
 
 // minimum over the K lanes of a group of 32-bit keys (same pairing as gsum)
@@ -1469,12 +1488,104 @@ __device__ __forceinline__ void lp_mlp_bf16(const LpLds<float, K>& L, int g, int
     for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLpB3 + r);
 }
 
+// FP32 walks, K >= 16: the hidden layers as BF16 matrix-core tiles (FP32
+// accumulation), the G <= 4 walkers of the wave the columns of 16 x 16 tiles
+// (V_MFMA_F32_16X16X32_BF16: result element r of lane t is unit 4q + r of
+// the tile, q = t / 16, column t % 16), 4 tiles of 16 units per layer; layer 1
+// contracts the 18 features (zero-padded to 32), layer 2 the 64 units in two
+// steps of 32.  Biases, tanh and the 4 outputs stay on the walker's lanes
+// (unit l + K j), reading the tiles' pre-activations through LDS.  Features
+// and activations are rounded to bfloat16 as in the dot-product form, and
+// both passes of a proposal evaluate this same function, so the
+// Metropolis-Hastings ratio stays exact.
+typedef __bf16 lp_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float lp_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ lp_f4 lp_tile(uint4 a, uint4 b, lp_f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(lp_bf16x8, a),
+                                                   __builtin_bit_cast(lp_bf16x8, b), c, 0, 0, 0);
+}
+
+template <int K>
+__device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int l,
+                                            const float (&f)[kLpIn], float (&o)[kLpOut])
+{
+    constexpr int G = 64 / K, UPL = kLpHid / K;
+    static_assert(G <= 16, "one tile column per walker");
+    const int t = threadIdx.x & 63, c = t & 15, q = t >> 4;
+    const int wv = c < G ? c : G - 1;
+    uint16_t* hb = reinterpret_cast<uint16_t*>(L.h);   // [G][64] bf16 activations
+    const uint32_t* W32 = reinterpret_cast<const uint32_t*>(L.W);
+    // the walker's features, bf16 pairs, zero-padded to 32
+    if (l == 0) {
+        uint32_t p[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++)
+            p[m] = m < kLpIn / 2 ? __builtin_bit_cast(uint32_t, lp_pack(f[2 * m], f[2 * m + 1])) : 0u;
+#pragma unroll
+        for (int m = 0; m < 16; m += 4)
+            *reinterpret_cast<uint4*>(L.fs + g * 16 + m) = make_uint4(p[m], p[m + 1], p[m + 2], p[m + 3]);
+    }
+    __syncthreads();
+    // component-wise selects (a select of whole uint4 values goes through scratch)
+    auto keep = [&](uint4 v) {
+        const bool k = c < G;
+        return make_uint4(k ? v.x : 0u, k ? v.y : 0u, k ? v.z : 0u, k ? v.w : 0u);
+    };
+    const uint4 b = keep(*reinterpret_cast<const uint4*>(L.fs + wv * 16 + 4 * q));   // features 8q .. 8q+7
+    // layer 1: W1 rows are 18 bf16 (36 B): four 4-byte words from index 8q,
+    // the ones past the 18th feature zero (q = 2: one word, q = 3: none)
+#pragma unroll
+    for (int tu = 0; tu < 4; tu++) {
+        const int u = 16 * tu + c;
+        const uint32_t* r = W32 + (kLpW1 + u * kLpIn) / 2 + 4 * (q < 3 ? q : 2);
+        const uint4 a = make_uint4(q < 3 ? r[0] : 0u, q < 2 ? r[1] : 0u, q < 2 ? r[2] : 0u,
+                                   q < 2 ? r[3] : 0u);
+        const lp_f4 d = lp_tile(a, b, (lp_f4){0.f, 0.f, 0.f, 0.f});
+        if (c < G) *reinterpret_cast<lp_f4*>(L.xa + c * kLpHid + 16 * tu + 4 * q) = d;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < UPL; j++) {
+        const int u = l + K * j;
+        const float a = L.xa[g * kLpHid + u] + L.w(kLpB1 + u);
+        hb[g * kLpHid + u] = __builtin_bit_cast(uint16_t, (__bf16)lp_tanh(a));
+    }
+    __syncthreads();
+    // layer 2: B = the walker's first-layer activations, two steps of 32 units
+    const uint4 b0 = keep(*reinterpret_cast<const uint4*>(hb + wv * kLpHid + 8 * q));
+    const uint4 b1 = keep(*reinterpret_cast<const uint4*>(hb + wv * kLpHid + 32 + 8 * q));
+#pragma unroll
+    for (int tu = 0; tu < 4; tu++) {
+        const uint16_t* row = L.W + kLpW2 + (16 * tu + c) * kLpHid;
+        lp_f4 d = lp_tile(*reinterpret_cast<const uint4*>(row + 8 * q), b0, (lp_f4){0.f, 0.f, 0.f, 0.f});
+        d = lp_tile(*reinterpret_cast<const uint4*>(row + 32 + 8 * q), b1, d);
+        if (c < G) *reinterpret_cast<lp_f4*>(L.xa + c * kLpHid + 16 * tu + 4 * q) = d;
+    }
+    __syncthreads();
+    float tt[kLpOut] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < UPL; j++) {
+        const int u = l + K * j;
+        const float h2 = lp_tanh(L.xa[g * kLpHid + u] + L.w(kLpB2 + u));
+#pragma unroll
+        for (int r = 0; r < kLpOut; r++) tt[r] += L.w(kLpW3 + r * kLpHid + u) * h2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(tt[r]) + L.w(kLpB3 + r);
+}
+
 template <typename R, int K>
 __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const R (&f)[kLpIn],
                                        R (&o)[kLpOut])
 {
     if constexpr (std::is_same<R, float>::value) {
-        lp_mlp_bf16<K>(L, g, l, f, o);
+        if constexpr (lp_mf(sizeof(R), K))
+            lp_mlp_mfma<K>(L, g, l, f, o);
+        else
+            lp_mlp_bf16<K>(L, g, l, f, o);
         return;
     }
     constexpr int UPL = kLpHid / K;
