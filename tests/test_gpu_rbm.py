@@ -206,7 +206,10 @@ def test_rbm_mfma_f32_retraces_lanes():
     """The FP32 matrix-core tiles add the products of each contraction in the
     order the lanes' FMA chains do (index 4e + q over the eight 16x16x4 steps
     is 0, 1, 2, ... ; the padding adds exact zeros), so the FP32 walk gives the
-    same energies, gradient sums and positions in both modes, bit for bit."""
+    same energies and positions in both modes, bit for bit.  The
+    gradient-window sums of dlnpsi/dW take FP32 products and 4-walker sums on
+    the tiles (FP64 products on the lanes) before the FP64 accumulation: they
+    agree to 1e-6 relative."""
     N, omega, kinds, H = 12, 1.0, RBMPJ, 24
     rng = np.random.default_rng(907)
     W, nd = 256, 24
@@ -222,5 +225,9 @@ def test_rbm_mfma_f32_retraces_lanes():
             sums, g, ge, _ = eng.run(300, 100, 0, 1)
             out[mode] = (np.array([sums[k] for k in ("energy", "energy2", "kinetic", "accepted")]),
                          g, ge, eng.positions())
-    for a, b in zip(out[1], out[2]):
-        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(out[1][0], out[2][0])   # energy sums, acceptance
+    np.testing.assert_array_equal(out[1][3], out[2][3])   # positions
+    for k in (1, 2):                                      # gradient sums
+        scale = np.max(np.abs(out[1][k]))
+        print(f"  gradient sums max |diff| / max {np.max(np.abs(out[1][k] - out[2][k])) / scale:.2e}")
+        np.testing.assert_allclose(out[2][k], out[1][k], rtol=1e-6, atol=1e-6 * scale)

@@ -1773,6 +1773,50 @@ __device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& 
     w.ln = L.n + (size_t)w.g * P.H;
 }
 
+// the gradient-window sums of dlnpsi/dW = xs n^T over the wave's walkers on
+// FP32 matrix-core tiles (V_MFMA_F32_16X16X4_F32): coordinates k are the tile
+// rows, hidden units h the columns, the walkers the contraction index (4 per
+// instruction); FP32 products and 4-walker sums, each step's tile added to
+// the FP64 LDS sums (every (k, h) belongs to one lane).  Reads the staged
+// current sorted vectors, n and E_L of the live walkers.
+template <int K>
+__device__ __forceinline__ void rbm_mf_wgrad(const WalkParams& P, const RbmLds<float, K>& L, int glive)
+{
+    constexpr int G = 64 / K, OPS = G / 4;
+    static_assert(G >= 4 && G <= 16, "4..16 walkers per wave");
+    const int t = threadIdx.x & 63, c = t & 15, q = t >> 4;
+    const int ND = P.ND, H = P.H;
+    // one tile at a time, its operands read from LDS where they are used (few
+    // registers live: the walk kernel is at its register cap)
+#pragma unroll
+    for (int tk = 0; tk < 2; tk++)
+#pragma unroll
+        for (int th = 0; th < 2; th++) {
+            if (16 * tk >= ND || 16 * th >= H) continue;   // wave-uniform
+            const int kr = 16 * tk + c, hc = 16 * th + c;
+            mf_f4 d1 = {0.f, 0.f, 0.f, 0.f}, d2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int op = 0; op < OPS; op++) {
+                const int gw = 4 * op + q;   // walker: the contraction index of this lane
+                const bool live = gw < glive;
+                const float x = L.xs[gw * 2 * ND + (kr < ND ? kr : ND - 1)];
+                const float n = L.n[gw * H + (hc < H ? hc : H - 1)];
+                const float xa = (live && kr < ND) ? x : 0.f;
+                const float nb = (live && hc < H) ? n : 0.f;
+                d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, nb, d1, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(float(L.E[gw]) * xa, nb, d2, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int k = 16 * tk + 4 * q + i;
+                if (k < ND && hc < H) {
+                    L.accW[k * H + hc] += double(d1[i]);
+                    L.accW[ND * H + k * H + hc] += double(d2[i]);
+                }
+            }
+        }
+}
+
 // ------------------------------------------------------------------------
 // the walk kernel: equil + steps Metropolis steps per walker
 // ------------------------------------------------------------------------
@@ -1970,10 +2014,18 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                             rGb[j] += double(w.rn[j]);
                             rGbE[j] += E * double(w.rn[j]);
                         }
-                        // dlnpsi/dW[k][h] = xs_k n_h, summed over the wave's walkers: lane
-                        // (kb, hb) of an 8 x 8 grid owns a block of (k, h) pairs
+                        // dlnpsi/dW[k][h] = xs_k n_h, summed over the wave's walkers
                         if (w.l == 0) L.E[w.g] = E;
                         __syncthreads();
+                        bool wg_done = false;
+                        if constexpr (sizeof(R) == 4 && K >= 4) {
+                            if (w.mf_on(P)) {
+                                rbm_mf_wgrad<K>(P, L, glive);
+                                wg_done = true;
+                            }
+                        }
+                        if (!wg_done) {
+                        // lane (kb, hb) of an 8 x 8 grid owns a block of (k, h) pairs
                         const int kbs = (P.ND + 7) >> 3, hbs = (P.H + 7) >> 3;
                         const int kb = lane >> 3, hb = lane & 7;
                         for (int kk = 0; kk < kbs; kk++) {
@@ -2001,6 +2053,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                                     L.accW[P.ND * P.H + kc * P.H + h] += s2[hh];
                                 }
                             }
+                        }
                         }
                         __syncthreads();
                     }
@@ -2093,10 +2146,16 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
 // FP32 RBM instantiations: capped at 256 registers for 2 waves/SIMD (they
 // need ~330; the spilled ones are cold enough that the second wave wins,
 // 1.2x at N=12, H=24)
-template <typename R, int NP, int K, bool RB, bool LP, bool DN>
+// MF: the RBM-product contractions on the matrix cores (P.rbm_mf != 0); the
+// instantiation for each case lets the compiler drop the other's code
+template <typename R, int NP, int K, bool RB, bool LP, bool DN, bool MF = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((RB && sizeof(R) == 4) ? 2 : 1)))
 void walk_kernel(const WalkParams P)
 {
+    if constexpr (MF)
+        __builtin_assume(P.rbm_mf == 1 || P.rbm_mf == 2);
+    else
+        __builtin_assume(P.rbm_mf == 0);
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     extern __shared__ double rbm_dyn[];
     WalkDyn D = walk_dyn(P);
@@ -2254,6 +2313,13 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
     const int blocks = (P.W + G - 1) / G;
     const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT), rbm_w_fits_static<RT, NP, K>())
                           : ((LP && !lp_fits_static<RT, NP, K>()) ? lp_lds_bytes(K, sizeof(RT)) : 0);
+    if constexpr (RB && sizeof(RT) == 4 && K >= 4) {
+        if (P.rbm_mf) {
+            hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, true>), dim3(blocks), dim3(64), lds,
+                               stream, P);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN>), dim3(blocks), dim3(64), lds, stream, P);
     return hipGetLastError();
 }
