@@ -2146,16 +2146,13 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
 // FP32 RBM instantiations: capped at 256 registers for 2 waves/SIMD (they
 // need ~330; the spilled ones are cold enough that the second wave wins,
 // 1.2x at N=12, H=24)
-// MF: the RBM-product contractions on the matrix cores (P.rbm_mf != 0); the
-// instantiation for each case lets the compiler drop the other's code
-template <typename R, int NP, int K, bool RB, bool LP, bool DN, bool MF = false>
+// MF: the RBM-product contraction mode P.rbm_mf (0 lanes, 1 FP32 tiles, 2
+// BF16 tiles); an instantiation per mode lets the compiler drop the others' code
+template <typename R, int NP, int K, bool RB, bool LP, bool DN, int MF = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((RB && sizeof(R) == 4) ? 2 : 1)))
 void walk_kernel(const WalkParams P)
 {
-    if constexpr (MF)
-        __builtin_assume(P.rbm_mf == 1 || P.rbm_mf == 2);
-    else
-        __builtin_assume(P.rbm_mf == 0);
+    __builtin_assume(P.rbm_mf == MF);
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     extern __shared__ double rbm_dyn[];
     WalkDyn D = walk_dyn(P);
@@ -2314,8 +2311,13 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
     const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT), rbm_w_fits_static<RT, NP, K>())
                           : ((LP && !lp_fits_static<RT, NP, K>()) ? lp_lds_bytes(K, sizeof(RT)) : 0);
     if constexpr (RB && sizeof(RT) == 4 && K >= 4) {
-        if (P.rbm_mf) {
-            hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, true>), dim3(blocks), dim3(64), lds,
+        if (P.rbm_mf == 1) {
+            hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 1>), dim3(blocks), dim3(64), lds,
+                               stream, P);
+            return hipGetLastError();
+        }
+        if (P.rbm_mf == 2) {
+            hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 2>), dim3(blocks), dim3(64), lds,
                                stream, P);
             return hipGetLastError();
         }
