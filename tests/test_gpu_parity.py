@@ -91,10 +91,14 @@ def scaled_errors(g0, r0, g1, r1):
     (12, O.IMPORTANCE_EXACT, 0.02, KINDS_SGP, 0.0),
     (12, O.IMPORTANCE_EXACT, 0.05, KINDS_SGP, 1.0),
     (20, O.IMPORTANCE_COMPAT, 0.01, KINDS_SGP, 0.0),
+    (30, O.IMPORTANCE_EXACT, 0.01, KINDS_SGP, 1.0),
+    (56, O.IMPORTANCE_EXACT, 0.005, KINDS_SGP, 1.0),
 ])
 def test_fp64_trajectories_match_oracle(N, sampling, dt, kinds, drift):
     """Same Philox streams, same positions: the FP64 walk must retrace the
-    oracle's chains step for step (accept decisions, positions, sums)."""
+    oracle's chains step for step (accept decisions, positions, sums).  The
+    exact-Langevin Slater x Gaussian x Pade-Jastrow cases with Coulomb run the
+    walk's instantiation specialised to that configuration."""
     omega = OMEGA_FOR_N[N]
     rng = np.random.default_rng(7 + N)
     W, steps, equil, seed = 64, 200, 20, 99 + N
@@ -173,7 +177,7 @@ def test_two_electron_dot_energy_is_three():
     assert sums["nonfinite"] == 0
 
 
-@pytest.mark.parametrize("N", [6, 12, 30])
+@pytest.mark.parametrize("N", [6, 12, 30, 56])
 def test_fp32_walk_agrees_with_fp64_walk(N):
     """The FP32 walk (FP64 inverse rebuild every 100 steps) samples the same
     distribution as the FP64 walk.  Both start from the same walkers and draw

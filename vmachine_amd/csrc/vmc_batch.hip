@@ -25,13 +25,19 @@ namespace vmc {
 // while the values that change between iterations (alpha, gamma, random
 // counter, gradient window) come from device memory, so that a captured
 // iteration can be replayed
-template <typename R, int NP, int K>
+// SG: Slater x Gaussian x Pade-Jastrow, Coulomb, exact Langevin, asserted to
+// the compiler (the single-system kernel's specialisation of the same name)
+template <typename R, int NP, int K, bool SG>
 __global__ __launch_bounds__(64) void batch_walk_kernel(const WalkParams P, int user, int t_begin,
                                                         int t_end, int acc_add,
                                                         const BatchState* __restrict__ st,
                                                         const double* __restrict__ theta)
 {
     constexpr int G = 64 / K;
+    if constexpr (SG) {
+        __builtin_assume(P.sampling == kImportanceExact);
+        __builtin_assume(P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1 && P.coulomb == 1);
+    }
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     WalkDyn D;
     D.alpha = theta[2 * user];
@@ -125,16 +131,21 @@ hipError_t launch_batch_walk(int precision, int shape, const WalkParams& P, int 
                              int t_end, int acc_add, const BatchState* st, const double* theta,
                              hipStream_t stream)
 {
+    const bool sg = P.sampling == kImportanceExact && P.has_slater == 1 && P.has_gauss == 1 &&
+                    P.has_pade == 1 && P.coulomb == 1;
     switch (shape) {
 #define X(id, np, k)                                                                               \
     case id: {                                                                                     \
         const int blocks = (P.W + 64 / k - 1) / (64 / k);                                          \
-        if (precision == 32)                                                                       \
-            hipLaunchKernelGGL((batch_walk_kernel<float, np, k>), dim3(blocks), dim3(64), 0,       \
+        if (precision == 32 && sg)                                                                 \
+            hipLaunchKernelGGL((batch_walk_kernel<float, np, k, true>), dim3(blocks), dim3(64), 0, \
+                               stream, P, user, t_begin, t_end, acc_add, st, theta);              \
+        else if (precision == 32)                                                                  \
+            hipLaunchKernelGGL((batch_walk_kernel<float, np, k, false>), dim3(blocks), dim3(64), 0,\
                                stream, P, user, t_begin, t_end, acc_add, st, theta);              \
         else                                                                                       \
-            hipLaunchKernelGGL((batch_walk_kernel<double, np, k>), dim3(blocks), dim3(64), 0,      \
-                               stream, P, user, t_begin, t_end, acc_add, st, theta);              \
+            hipLaunchKernelGGL((batch_walk_kernel<double, np, k, false>), dim3(blocks), dim3(64),  \
+                               0, stream, P, user, t_begin, t_end, acc_add, st, theta);           \
         break;                                                                                     \
     }
         VMC_BATCH_SHAPES(X)

@@ -2148,11 +2148,20 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
 // 1.2x at N=12, H=24)
 // MF: the RBM-product contraction mode P.rbm_mf (0 lanes, 1 FP32 tiles, 2
 // BF16 tiles); an instantiation per mode lets the compiler drop the others' code
-template <typename R, int NP, int K, bool RB, bool LP, bool DN, int MF = 0>
+// SG: the walk of Slater x Gaussian x Pade-Jastrow with Coulomb interaction
+// and exact Langevin moves (learned moves with LP), the configuration of the
+// quantum-dot studies, asserted to the compiler so that the other elements'
+// and samplers' code drops out of this instantiation
+template <typename R, int NP, int K, bool RB, bool LP, bool DN, int MF = 0, bool SG = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((RB && sizeof(R) == 4) ? 2 : 1)))
 void walk_kernel(const WalkParams P)
 {
     __builtin_assume(P.rbm_mf == MF);
+    if constexpr (SG) {
+        __builtin_assume(P.sampling == (LP ? kLearned : kImportanceExact));
+        __builtin_assume(P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1);
+        __builtin_assume(P.coulomb == 1 && P.has_rbmg == 0 && P.has_rbmp == 0);
+    }
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     extern __shared__ double rbm_dyn[];
     WalkDyn D = walk_dyn(P);
@@ -2303,7 +2312,7 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
 // ------------------------------------------------------------------------
 using RT = std::conditional_t<VMC_WALK_PRECISION == 32, float, double>;
 
-template <int NP, int K, bool RB, bool LP, bool DN>
+template <int NP, int K, bool RB, bool LP, bool DN, bool SG = false>
 static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
 {
     constexpr int G = 64 / K;
@@ -2322,7 +2331,8 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN>), dim3(blocks), dim3(64), lds, stream, P);
+    hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 0, SG>), dim3(blocks), dim3(64), lds,
+                       stream, P);
     return hipGetLastError();
 }
 
@@ -2360,20 +2370,28 @@ hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, bool lp, const WalkP
 {
     if (lp) {
         if (rbm) return hipErrorInvalidValue;
+    const bool sgl = P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1 && P.coulomb == 1;
 #define Y(a, b)                                                                       \
-    if (np == a && K == b)                                                            \
-        return P.dens_flags ? launch_walk_t<a, b, false, true, true>(P, stream)       \
-                            : launch_walk_t<a, b, false, true, false>(P, stream);
+    if (np == a && K == b) {                                                          \
+        if (P.dens_flags) return launch_walk_t<a, b, false, true, true>(P, stream);   \
+        if (sgl) return launch_walk_t<a, b, false, true, false, true>(P, stream);     \
+        return launch_walk_t<a, b, false, true, false>(P, stream);                    \
+    }
         VMC_LP_SHAPES(Y)
 #undef Y
         return hipErrorInvalidValue;
     }
     // every shape has an instantiation without the density code (its registers
-    // and scheduling stay out of the walks that count no densities)
-#define X(a, b, c)                                                              \
-    if (np == a && K == b && rbm == c)                                          \
-        return P.dens_flags ? launch_walk_t<a, b, c, false, true>(P, stream)    \
-                            : launch_walk_t<a, b, c, false, false>(P, stream);
+    // and scheduling stay out of the walks that count no densities), and the
+    // plain shapes one for Slater x Gaussian x Pade-Jastrow, Coulomb, exact Langevin
+    const bool sg = P.sampling == kImportanceExact && P.has_slater == 1 && P.has_gauss == 1 &&
+                    P.has_pade == 1 && P.coulomb == 1 && P.has_rbmg == 0 && P.has_rbmp == 0;
+#define X(a, b, c)                                                                      \
+    if (np == a && K == b && rbm == c) {                                                \
+        if (P.dens_flags) return launch_walk_t<a, b, c, false, true>(P, stream);        \
+        if (!c && sg) return launch_walk_t<a, b, c, false, false, true>(P, stream);     \
+        return launch_walk_t<a, b, c, false, false>(P, stream);                         \
+    }
     VMC_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
