@@ -47,7 +47,7 @@ def bytes_per_step(N, n, b, D=2, a=1.0):
 # HBM bytes the FP32 N=56 walk kernel moves per walker per launch (positions in
 # and out, accumulators), from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by
 # a 1 GiB copy (profiles/walk_kernel_f32_N56_w0.1_hbm_traffic.md)
-PMC_BYTES_PER_WALKER_LAUNCH = 2116.0
+PMC_BYTES_PER_WALKER_LAUNCH = 2090.0
 
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
