@@ -2148,17 +2148,18 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
 // 1.2x at N=12, H=24)
 // MF: the RBM-product contraction mode P.rbm_mf (0 lanes, 1 FP32 tiles, 2
 // BF16 tiles); an instantiation per mode lets the compiler drop the others' code
-// SG: the walk of Slater x Gaussian x Pade-Jastrow with Coulomb interaction
-// and exact Langevin moves (learned moves with LP), the configuration of the
-// quantum-dot studies, asserted to the compiler so that the other elements'
-// and samplers' code drops out of this instantiation
-template <typename R, int NP, int K, bool RB, bool LP, bool DN, int MF = 0, bool SG = false>
+// SG >= 0: the walk of Slater x Gaussian x Pade-Jastrow with Coulomb
+// interaction and sampler SG (exact Langevin, the reference's importance
+// sampling, or learned moves with LP), the configurations of the quantum-dot
+// studies, asserted to the compiler so that the other elements' and
+// samplers' code drops out of the instantiation; SG = -1: any configuration
+template <typename R, int NP, int K, bool RB, bool LP, bool DN, int MF = 0, int SG = -1>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((RB && sizeof(R) == 4) ? 2 : 1)))
 void walk_kernel(const WalkParams P)
 {
     __builtin_assume(P.rbm_mf == MF);
-    if constexpr (SG) {
-        __builtin_assume(P.sampling == (LP ? kLearned : kImportanceExact));
+    if constexpr (SG >= 0) {
+        __builtin_assume(P.sampling == SG);
         __builtin_assume(P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1);
         __builtin_assume(P.coulomb == 1 && P.has_rbmg == 0 && P.has_rbmp == 0);
     }
@@ -2312,7 +2313,7 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
 // ------------------------------------------------------------------------
 using RT = std::conditional_t<VMC_WALK_PRECISION == 32, float, double>;
 
-template <int NP, int K, bool RB, bool LP, bool DN, bool SG = false>
+template <int NP, int K, bool RB, bool LP, bool DN, int SG = -1>
 static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
 {
     constexpr int G = 64 / K;
@@ -2365,6 +2366,21 @@ static hipError_t launch_eval_t(const WalkParams& P, const double* pos, const in
 // shapes with the learned proposal (no RBM elements)
 #define VMC_LP_SHAPES(X) X(1, 1) X(3, 4) X(6, 8) X(10, 16) X(15, 16) X(21, 32) X(28, 32)
 
+// one shape's launch: density counting, then the specialised samplers of the
+// plain shapes, then the generic instantiation
+template <int NP, int K, bool RB>
+static hipError_t launch_walk_shape(const WalkParams& P, bool sg, hipStream_t stream)
+{
+    if (P.dens_flags) return launch_walk_t<NP, K, RB, false, true>(P, stream);
+    if constexpr (!RB) {
+        if (sg && P.sampling == kImportanceExact)
+            return launch_walk_t<NP, K, RB, false, false, kImportanceExact>(P, stream);
+        if (sg && P.sampling == kImportanceCompat)
+            return launch_walk_t<NP, K, RB, false, false, kImportanceCompat>(P, stream);
+    }
+    return launch_walk_t<NP, K, RB, false, false>(P, stream);
+}
+
 hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, bool lp, const WalkParams& P,
                                    hipStream_t stream)
 {
@@ -2374,7 +2390,7 @@ hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, bool lp, const WalkP
 #define Y(a, b)                                                                       \
     if (np == a && K == b) {                                                          \
         if (P.dens_flags) return launch_walk_t<a, b, false, true, true>(P, stream);   \
-        if (sgl) return launch_walk_t<a, b, false, true, false, true>(P, stream);     \
+        if (sgl) return launch_walk_t<a, b, false, true, false, kLearned>(P, stream); \
         return launch_walk_t<a, b, false, true, false>(P, stream);                    \
     }
         VMC_LP_SHAPES(Y)
@@ -2383,15 +2399,12 @@ hipError_t VMC_SUFFIX(launch_walk)(int np, int K, bool rbm, bool lp, const WalkP
     }
     // every shape has an instantiation without the density code (its registers
     // and scheduling stay out of the walks that count no densities), and the
-    // plain shapes one for Slater x Gaussian x Pade-Jastrow, Coulomb, exact Langevin
-    const bool sg = P.sampling == kImportanceExact && P.has_slater == 1 && P.has_gauss == 1 &&
-                    P.has_pade == 1 && P.coulomb == 1 && P.has_rbmg == 0 && P.has_rbmp == 0;
-#define X(a, b, c)                                                                      \
-    if (np == a && K == b && rbm == c) {                                                \
-        if (P.dens_flags) return launch_walk_t<a, b, c, false, true>(P, stream);        \
-        if (!c && sg) return launch_walk_t<a, b, c, false, false, true>(P, stream);     \
-        return launch_walk_t<a, b, c, false, false>(P, stream);                         \
-    }
+    // plain shapes ones for Slater x Gaussian x Pade-Jastrow with Coulomb under
+    // exact Langevin and under the reference's importance sampling
+    const bool sg = P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1 && P.coulomb == 1 &&
+                    P.has_rbmg == 0 && P.has_rbmp == 0;
+#define X(a, b, c) \
+    if (np == a && K == b && rbm == c) return launch_walk_shape<a, b, c>(P, sg, stream);
     VMC_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
