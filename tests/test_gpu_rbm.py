@@ -202,23 +202,26 @@ def test_rbm_fp32_walk_agrees_with_fp64(rbm_matrix):
     assert abs(a32 - a64) < 0.01
 
 
-def test_rbm_mfma_f32_retraces_lanes():
+@pytest.mark.parametrize("sampling", [O.IMPORTANCE_EXACT, O.IMPORTANCE_COMPAT])
+def test_rbm_mfma_f32_retraces_lanes(sampling):
     """The FP32 matrix-core tiles add the products of each contraction in the
     order the lanes' FMA chains do (index 4e + q over the eight 16x16x4 steps
     is 0, 1, 2, ... ; the padding adds exact zeros), so the FP32 walk gives the
     same energies and positions in both modes, bit for bit.  The
     gradient-window sums of dlnpsi/dW take FP32 products and 4-walker sums on
     the tiles (FP64 products on the lanes) before the FP64 accumulation: they
-    agree to 1e-6 relative."""
+    agree to 1e-6 relative.  Under the reference's importance sampling the
+    FP32-tile walk of Slater x RBM-Gaussian x RBM-product x Pade-Jastrow runs
+    the instantiation specialised to that element set."""
     N, omega, kinds, H = 12, 1.0, RBMPJ, 24
     rng = np.random.default_rng(907)
     W, nd = 256, 24
     P = rbm_parameters(rng, kinds, nd, H)
     out = {}
     for mode in (1, 2):
-        _, ecfg = _configs(N, omega, kinds, H, True, REF, O.IMPORTANCE_EXACT, 0.02, 32, W, mode)
+        _, ecfg = _configs(N, omega, kinds, H, True, REF, sampling, 0.02, 32, W, mode)
         ecfg.refresh_interval = 50
-        print(f"rbm fp32 walk rbm_matrix={mode}", flush=True)
+        print(f"rbm fp32 walk rbm_matrix={mode} sampling={sampling}", flush=True)
         with _engine(ecfg) as eng:
             eng.set_parameters(P)
             eng.init_walkers(11)

@@ -2158,10 +2158,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((RB && sizeo
 void walk_kernel(const WalkParams P)
 {
     __builtin_assume(P.rbm_mf == MF);
-    if constexpr (SG >= 0) {
+    if constexpr (SG >= 0 && !RB) {
         __builtin_assume(P.sampling == SG);
         __builtin_assume(P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1);
         __builtin_assume(P.coulomb == 1 && P.has_rbmg == 0 && P.has_rbmp == 0);
+    }
+    if constexpr (SG >= 0 && RB) {   // Slater x RBM-Gaussian x RBM-product x Pade-Jastrow
+        __builtin_assume(P.sampling == SG);
+        __builtin_assume(P.has_slater == 1 && P.has_rbmg == 1 && P.has_rbmp == 1 && P.has_pade == 1);
+        __builtin_assume(P.coulomb == 1 && P.has_gauss == 0);
     }
     __shared__ double lds[walk_static_lds_doubles<NP, K>()];
     extern __shared__ double rbm_dyn[];
@@ -2321,6 +2326,17 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
     const size_t lds = RB ? rbm_lds_bytes(K, P.ND, P.H, sizeof(RT), rbm_w_fits_static<RT, NP, K>())
                           : ((LP && !lp_fits_static<RT, NP, K>()) ? lp_lds_bytes(K, sizeof(RT)) : 0);
     if constexpr (RB && sizeof(RT) == 4 && K >= 4) {
+        // the RBM x Slater x Pade-Jastrow walk under the reference's importance
+        // sampling with the default FP32 tiles has an instantiation of its own
+        if constexpr (!DN) {
+            if (P.rbm_mf == 1 && P.sampling == kImportanceCompat && P.has_slater == 1 &&
+                P.has_rbmg == 1 && P.has_rbmp == 1 && P.has_pade == 1 && P.coulomb == 1 &&
+                P.has_gauss == 0) {
+                hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 1, kImportanceCompat>),
+                                   dim3(blocks), dim3(64), lds, stream, P);
+                return hipGetLastError();
+            }
+        }
         if (P.rbm_mf == 1) {
             hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 1>), dim3(blocks), dim3(64), lds,
                                stream, P);
