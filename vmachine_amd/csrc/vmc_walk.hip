@@ -2327,12 +2327,17 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
                           : ((LP && !lp_fits_static<RT, NP, K>()) ? lp_lds_bytes(K, sizeof(RT)) : 0);
     if constexpr (RB && sizeof(RT) == 4 && K >= 4) {
         // the RBM x Slater x Pade-Jastrow walk under the reference's importance
-        // sampling with the default FP32 tiles has an instantiation of its own
+        // sampling with the tiles has instantiations of its own
         if constexpr (!DN) {
-            if (P.rbm_mf == 1 && P.sampling == kImportanceCompat && P.has_slater == 1 &&
-                P.has_rbmg == 1 && P.has_rbmp == 1 && P.has_pade == 1 && P.coulomb == 1 &&
-                P.has_gauss == 0) {
+            const bool set = P.sampling == kImportanceCompat && P.has_slater == 1 && P.has_rbmg == 1 &&
+                             P.has_rbmp == 1 && P.has_pade == 1 && P.coulomb == 1 && P.has_gauss == 0;
+            if (set && P.rbm_mf == 1) {
                 hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 1, kImportanceCompat>),
+                                   dim3(blocks), dim3(64), lds, stream, P);
+                return hipGetLastError();
+            }
+            if (set && P.rbm_mf == 2) {
+                hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 2, kImportanceCompat>),
                                    dim3(blocks), dim3(64), lds, stream, P);
                 return hipGetLastError();
             }
