@@ -5,7 +5,7 @@
 // drives the reference's System, wave-function elements, Metropolis samplers,
 // Sampler and optimisers with scripted random numbers, so that the numbers it
 // prints are the reference's answers for inputs chosen by the test generator
-// (tests/golden/make_golden.py).  Nothing here runs on the GPU box.
+// (tests/golden/make_golden.py).  Nothing here runs with the GPU tests.
 //
 // usage: vmc_ref_harness <input.in> <script> <out.json>
 //   script lines (whitespace separated):

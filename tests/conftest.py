@@ -7,5 +7,5 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on a machine with one)")
     config.addinivalue_line("markers", "slow: longer statistical runs")
