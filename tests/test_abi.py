@@ -16,6 +16,19 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), f"libvmc_hip.so does not export {n}"
 
 
+def test_integration_index_lists_every_declared_symbol():
+    """The maintainer's index of entry points covers both headers."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    index = open(os.path.join(root, "INTEGRATION.md")).read().split("## 4. Index of entry points")
+    assert len(index) == 2, "INTEGRATION.md has no entry-point index"
+    listed = set(re.findall(r"`(vmc_\w+)`", index[1]))
+    host = open(os.path.join(root, "include", "vmc_host.h")).read()
+    declared = set(E.header_functions()) | set(re.findall(r"\b(vmc_host_\w+)\s*\(", host))
+    assert declared - listed == set(), sorted(declared - listed)
+
+
 def test_parameter_shapes_match_reference():
     c = V.make_config(56, omega=0.1)
     L = E.lib()
