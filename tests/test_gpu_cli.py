@@ -219,3 +219,27 @@ def test_output_files(tmp_path):
     assert (tmp_path / "weights_0.dat").exists()
     W = np.loadtxt(tmp_path / "weights_0.dat")
     assert W.shape == (2,)
+
+
+def test_fewer_steps_per_walker_than_gradient_batches_exits_143():
+    """ADAM averages its gradient over 10 windows of the walker's steps
+    (ref src/Optimization/adam.h:15, src/sampler.cpp:82-86); a walker count
+    that leaves fewer than 10 steps per walker is a configuration error, not
+    a run whose gradient divides by zero samples."""
+    r = H.run(["--walkers", "20000", os.path.join(EX, "small_fermionic_quantumdot_vmc.in")],
+              timeout=120)
+    assert r.returncode == 143, (r.returncode, r.stderr[-500:])
+    assert "gradient batches" in r.stderr
+
+
+def test_engine_rejects_more_batches_than_steps():
+    import vmachine_amd as V
+    from oracle import oracle as O
+    from tests.helpers import engine_config
+    cfg = engine_config(2, 1.0, [O.GAUSSIAN, O.PADE], O.IMPORTANCE_COMPAT, 0.05, walkers=8)
+    with V.Engine(cfg) as eng:
+        eng.set_parameters(np.array([[1.0], [1.0]]))
+        eng.init_walkers(1)
+        with pytest.raises(V.VmcError, match="batches must not exceed steps"):
+            eng.run(5, 0, 0, 10)
+        eng.run(10, 0, 0, 10)

@@ -181,9 +181,11 @@ def test_two_electron_dot_energy_is_three():
 def test_fp32_walk_agrees_with_fp64_walk(N):
     """The FP32 walk (FP64 inverse rebuild every 100 steps) samples the same
     distribution as the FP64 walk.  Both start from the same walkers and draw
-    the same Philox streams, so the chains stay close for a while and then
-    decorrelate; the energies must agree within their joint statistical error
-    (errors from the spread of per-walker means, independent chains)."""
+    the same Philox streams, so walker w of one precision is paired with
+    walker w of the other: the chains stay close for a while and then
+    decorrelate, and the per-walker differences of the mean local energy are
+    independent across walkers.  Their mean must be within 4 standard errors
+    of zero (the standard error from the spread of the differences)."""
     omega = OMEGA_FOR_N[N]
     W = 8192
     res = {}
@@ -196,12 +198,41 @@ def test_fp32_walk_agrees_with_fp64_walk(N):
             eng.run(3000, 0)
             sums, _, _, en = eng.run(2048, 0, record_walkers=W)
         means = en.astype(np.float64).mean(axis=1)
-        E = sums["energy"] / sums["samples"]
-        res[prec] = (E, means.std() / np.sqrt(W), en.min(), en.max())
-        print(f"N={N} precision={prec}: E = {E:.6f} +- {res[prec][1]:.2e} "
+        res[prec] = (sums["energy"] / sums["samples"], means)
+        print(f"N={N} precision={prec}: E = {res[prec][0]:.6f} "
               f"(E_L range {en.min():.2f} .. {en.max():.2f})")
-    d = abs(res[32][0] - res[64][0])
-    assert d < 4.0 * np.hypot(res[32][1], res[64][1]), res
+    d = res[32][1] - res[64][1]
+    se = d.std() / np.sqrt(W)
+    print(f"  paired difference {d.mean():.3e} +- {se:.2e} "
+          f"(independent-chain error {np.hypot(res[32][1].std(), res[64][1].std()) / np.sqrt(W):.2e})")
+    assert abs(d.mean()) < 4.0 * se + 1e-7 * abs(res[64][0]), (d.mean(), se)
+    assert abs(res[32][0] - res[64][0]) < 4.0 * se + 1e-6 * abs(res[64][0])
+
+
+@pytest.mark.parametrize("N", [2, 12, 56])
+def test_init_walkers_equal_oracle_initial_state(N):
+    """vmc_init_walkers draws the reference's RandomNormal initial state
+    x ~ N(0, 1/omega) (standard deviation 1/omega, ref
+    src/InitialStates/randomnormal.cpp:17) from the same Philox words as the
+    oracle's vmco_initial_positions, whose law is pinned on the CPU
+    (test_oracle_pins.py).  The Philox words are integers and equal; the
+    device's log, sqrt, cos and sin and the host C library's each round to
+    within an ulp or two, so the positions agree to a few ulps (most bit for
+    bit)."""
+    omega = OMEGA_FOR_N[N]
+    W, seed = 1024, 4242 + N
+    with _engine(engine_config(N, omega, KINDS_SGP, O.IMPORTANCE_COMPAT, 0.01,
+                               walkers=W)) as eng:
+        eng.init_walkers(seed)
+        g = eng.positions()
+    o = np.stack([O.initial_positions(seed, w, 2 * N, omega) for w in range(W)])
+    scale = np.sqrt(-2.0 * np.log(2.0 ** -24)) / omega   # largest |x| the draw can give
+    ulps = np.abs(g - o) / np.spacing(np.maximum(np.abs(o), 1e-3 * scale))
+    same = np.mean(g == o)
+    print(f"N={N}: {same:.4f} of the coordinates bit for bit, max {ulps.max():.1f} ulp")
+    assert ulps.max() <= 8.0
+    assert same > 0.5
+    assert abs(o.std() * omega - 1.0) < 0.05
 
 
 @pytest.mark.parametrize("precision", [64, 32])

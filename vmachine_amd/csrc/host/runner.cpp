@@ -346,6 +346,13 @@ int run_simulation(const Config& c, const RunOptions& o, std::ostream& out, std:
     Optimizer opt;
     opt.init(c.optimization, rows * cols, c.learningRate, c.optGamma, c.optMonotonic);
     const int B = opt.batches();
+    if (std::max(1, total0 / W) < B) {
+        // every gradient window needs at least one step per walker
+        err << "numSteps / walkers = " << std::max(1, total0 / W) << " steps per walker is fewer "
+            << "than the optimiser's " << B << " gradient batches: use fewer walkers" << std::endl;
+        vmc_engine_destroy(eng);
+        return 143;
+    }
 
     // ---- header: the reference's logo and information blocks with this run's
     // values (ref src/printinfo.cpp:14-140, src/system.cpp:83-91) ----
@@ -446,7 +453,9 @@ int run_simulation(const Config& c, const RunOptions& o, std::ostream& out, std:
             total = stepsInit * ratio;
         }
         const int S = std::max(1, total / W);
-        const int equil = int(S * c.equilibrationFraction);
+        // burn-in is fixed once from the unscaled steps (ref src/userinput.cpp:74):
+        // adaptive iterations lengthen only the sampled part (ref src/system.cpp:94-97)
+        const int equil = int(std::max(1, stepsInit / W) * c.equilibrationFraction);
         vmc_sums s;
         std::vector<double> g((size_t)rows * std::max(cols, 1)), ge(g.size());
         int rc;

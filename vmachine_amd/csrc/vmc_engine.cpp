@@ -512,6 +512,8 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
     if (!e->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
     if (steps < 1 || equil < 0 || batches < 1 || iteration < 0)
         return fail(VMC_E_INVALID, "steps >= 1, equilibration >= 0, batches >= 1 required");
+    if (batches > steps)   // an empty gradient window would divide by zero samples
+        return fail(VMC_E_INVALID, "batches must not exceed steps (each gradient window needs a step)");
     if (e->cfg.sampling == VMC_LEARNED && !e->lp_set)
         return fail(VMC_E_STATE, "the learned proposal has no weights (vmc_lp_set)");
     if (rec_walkers < 0 || rec_walkers > e->cfg.num_walkers)
