@@ -286,6 +286,23 @@ int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int e
                  const vmc_optimizer_config* opt, int use_graph, double* history,
                  double* kernel_ms);
 
+/* As vmc_optimize with use_graph = 0, allocating no device memory: no
+ * counterpart in the reference.  For callers that cannot allocate once the
+ * engine exists (tight optimisation loops, harnesses that forbid hipMalloc).
+ * The loop's device state (parameter matrix, optimiser moments, iteration
+ * state, history staging) lives in one file-scope array per GPU, set up on the
+ * first call on that GPU and held until the process ends, so a call accepts at
+ * most VMC_STATIC_MAX_PARAMS parameter-matrix entries (num_elements x max
+ * parameters) and VMC_STATIC_MAX_ITERATIONS iterations (VMC_E_INVALID
+ * beyond).  Every call on a GPU reuses that GPU's array: calls that overlap on
+ * one GPU (two threads, two engines) race and must be serialised by the
+ * caller.  The engine's own buffers come from vmc_engine_create. */
+#define VMC_STATIC_MAX_PARAMS 8192
+#define VMC_STATIC_MAX_ITERATIONS 1024
+int vmc_optimize_static(vmc_engine* e, double* params, int iterations, int steps,
+                        int equilibration, const vmc_optimizer_config* opt, double* history,
+                        double* kernel_ms);
+
 /* ---- Batches: several quantum dots walked side by side ----
  * Systems of different particle numbers and frequencies (Slater / Gaussian /
  * Pade-Jastrow elements, the same for all) are walked by one kernel each, the

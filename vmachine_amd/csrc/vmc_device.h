@@ -170,6 +170,21 @@ struct OptLayout {
 };
 enum : int { kOptAlpha = kNumWalkerAcc, kOptGamma, kOptOut };
 
+// allocation-free optimisation loop (vmc_optimize_static): its device state
+// lives in one file-scope array per GPU, laid out as
+// theta [max params] | m [max params] | v [max params] | alpha, gamma (4) |
+// OptState (4) | history [max iterations][kOptOut]
+constexpr int kStaticMaxParams = 8192;       // VMC_STATIC_MAX_PARAMS
+constexpr int kStaticMaxIterations = 1024;   // VMC_STATIC_MAX_ITERATIONS
+constexpr int kStaticTheta = 0;
+constexpr int kStaticM = kStaticTheta + kStaticMaxParams;
+constexpr int kStaticV = kStaticM + kStaticMaxParams;
+constexpr int kStaticAG = kStaticV + kStaticMaxParams;
+constexpr int kStaticState = kStaticAG + 4;
+constexpr int kStaticHist = kStaticState + 4;
+constexpr int kStaticDoubles = kStaticHist + kStaticMaxIterations * kOptOut;
+static_assert(sizeof(OptState) <= 4 * sizeof(double), "OptState fits its slot");
+
 // ---- batch of systems walked side by side (vmc_batch.hip) ----
 enum : int { kBatchAlpha = kNumWalkerAcc, kBatchGamma, kBatchOut };
 

@@ -16,6 +16,10 @@
 #include "vmc.h"
 #include "vmc_device.h"
 
+static_assert(vmc::kStaticMaxParams == VMC_STATIC_MAX_PARAMS &&
+                  vmc::kStaticMaxIterations == VMC_STATIC_MAX_ITERATIONS,
+              "the header's limits of vmc_optimize_static are the scratch array's");
+
 namespace vmc {
 hipError_t launch_walk(int precision, const KernelShape& sh, const WalkParams& P,
                        hipStream_t stream);
@@ -50,6 +54,8 @@ int batch_shape_id(int np, int K);
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
                                  hipStream_t stream);
+hipError_t launch_fill(double* p, int n, double value, hipStream_t stream);
+hipError_t static_scratch(double** out);
 hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int nd,
                         int n, int* k, double* z, double* um, double* ua, hipStream_t stream);
 }  // namespace vmc
@@ -619,8 +625,13 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
     return VMC_OK;
 }
 
-int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int equilibration,
-                 const vmc_optimizer_config* opt, int use_graph, double* history, double* kernel_ms)
+// One implementation of the device optimisation loop for both entry points:
+// vmc_optimize allocates the loop's device state (optimiser moments, state,
+// history staging) per call; vmc_optimize_static (use_static) takes it from
+// the per-device array of vmc_optim.hip and allocates nothing.
+static int optimize_impl(vmc_engine* e, double* params, int iterations, int steps,
+                         int equilibration, const vmc_optimizer_config* opt, int use_graph,
+                         double* history, double* kernel_ms, bool use_static)
 {
     if (!e || !params || !opt) return fail(VMC_E_INVALID, "null argument");
     if (!e->walkers_set) return fail(VMC_E_STATE, "walkers have no positions yet");
@@ -632,6 +643,12 @@ int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int e
     if (opt->kind < VMC_OPT_ADAM || opt->kind > VMC_OPT_GD || !(opt->learning_rate >= 0))
         return fail(VMC_E_INVALID, "optimizer kind VMC_OPT_ADAM/SGD/GD, learning_rate >= 0");
     const int E = e->cfg.num_elements, mp = e->maxpar, n = E * mp;
+    if (use_static && (n > vmc::kStaticMaxParams || iterations > vmc::kStaticMaxIterations))
+        return fail(VMC_E_INVALID,
+                    "vmc_optimize_static: at most " + std::to_string(vmc::kStaticMaxParams) +
+                        " parameter-matrix entries and " +
+                        std::to_string(vmc::kStaticMaxIterations) + " iterations per call (got " +
+                        std::to_string(n) + " and " + std::to_string(iterations) + ")");
     if (int rc = vmc_set_parameters(e, params, E, mp)) return rc;
     vmc::OptLayout L{};
     L.rows = E;
@@ -657,26 +674,51 @@ int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int e
     auto done = [&](int code) {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* p : {(void*)d_theta, (void*)d_m, (void*)d_v, (void*)d_ag, (void*)d_hist, (void*)d_st})
-            if (p) (void)hipFree(p);
+        if (!use_static)
+            for (void* p : {(void*)d_theta, (void*)d_m, (void*)d_v, (void*)d_ag, (void*)d_hist,
+                            (void*)d_st})
+                if (p) (void)hipFree(p);
         return code;
     };
-    const std::vector<double> ones(std::max(n, 1), 1.0);   // ref adam.cpp:17-19, sgd.cpp:20
     const double ag[2] = {e->alpha, e->gamma};
     const vmc::OptState st0{e->counter, 0, opt->batches, e->init_qf ? 1 : 0, 0};
     hipError_t err;
-    if ((err = hipMalloc(&d_theta, sizeof(double) * std::max(n, 1))) != hipSuccess ||
-        (err = hipMalloc(&d_m, sizeof(double) * std::max(n, 1))) != hipSuccess ||
-        (err = hipMalloc(&d_v, sizeof(double) * std::max(n, 1))) != hipSuccess ||
-        (err = hipMalloc(&d_ag, sizeof(ag))) != hipSuccess ||
-        (err = hipMalloc(&d_st, sizeof(st0))) != hipSuccess ||
-        (err = hipMalloc(&d_hist, sizeof(double) * (size_t)iterations * vmc::kOptOut)) != hipSuccess ||
-        (n > 0 && (err = hipMemcpy(d_theta, params, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) ||
-        (err = hipMemcpy(d_m, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (err = hipMemcpy(d_v, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (err = hipMemcpy(d_ag, ag, sizeof(ag), hipMemcpyHostToDevice)) != hipSuccess ||
-        (err = hipMemcpy(d_st, &st0, sizeof(st0), hipMemcpyHostToDevice)) != hipSuccess)
-        return done(fail(VMC_E_DEVICE, std::string("optimize set-up: ") + hipGetErrorString(err)));
+    if (use_static) {
+        double* s = nullptr;
+        if ((err = hipSetDevice(e->device)) != hipSuccess || (err = vmc::static_scratch(&s)) != hipSuccess)
+            return fail(VMC_E_DEVICE, std::string("vmc_optimize_static scratch: ") + hipGetErrorString(err));
+        d_theta = s + vmc::kStaticTheta;
+        d_m = s + vmc::kStaticM;
+        d_v = s + vmc::kStaticV;
+        d_ag = s + vmc::kStaticAG;
+        d_st = reinterpret_cast<vmc::OptState*>(s + vmc::kStaticState);
+        d_hist = s + vmc::kStaticHist;
+        // the array keeps the previous call's contents: every value the loop
+        // reads is written here first (the history rows are written by the
+        // iterations before they are copied out)
+        if ((n > 0 && (err = hipMemcpyAsync(d_theta, params, sizeof(double) * n, hipMemcpyHostToDevice,
+                                            e->stream)) != hipSuccess) ||
+            (err = vmc::launch_fill(d_m, n, 1.0, e->stream)) != hipSuccess ||   // ref adam.cpp:17-19
+            (err = vmc::launch_fill(d_v, n, 1.0, e->stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(d_ag, ag, sizeof(ag), hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(d_st, &st0, sizeof(st0), hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+            (err = hipStreamSynchronize(e->stream)) != hipSuccess)
+            return done(fail(VMC_E_DEVICE, std::string("optimize set-up: ") + hipGetErrorString(err)));
+    } else {
+        const std::vector<double> ones(std::max(n, 1), 1.0);   // ref adam.cpp:17-19, sgd.cpp:20
+        if ((err = hipMalloc(&d_theta, sizeof(double) * std::max(n, 1))) != hipSuccess ||
+            (err = hipMalloc(&d_m, sizeof(double) * std::max(n, 1))) != hipSuccess ||
+            (err = hipMalloc(&d_v, sizeof(double) * std::max(n, 1))) != hipSuccess ||
+            (err = hipMalloc(&d_ag, sizeof(ag))) != hipSuccess ||
+            (err = hipMalloc(&d_st, sizeof(st0))) != hipSuccess ||
+            (err = hipMalloc(&d_hist, sizeof(double) * (size_t)iterations * vmc::kOptOut)) != hipSuccess ||
+            (n > 0 && (err = hipMemcpy(d_theta, params, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) ||
+            (err = hipMemcpy(d_m, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+            (err = hipMemcpy(d_v, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+            (err = hipMemcpy(d_ag, ag, sizeof(ag), hipMemcpyHostToDevice)) != hipSuccess ||
+            (err = hipMemcpy(d_st, &st0, sizeof(st0), hipMemcpyHostToDevice)) != hipSuccess)
+            return done(fail(VMC_E_DEVICE, std::string("optimize set-up: ") + hipGetErrorString(err)));
+    }
 
     vmc::WalkParams P = base_params(e);
     P.steps = steps;
@@ -721,15 +763,15 @@ int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int e
         err = use_graph ? hipGraphLaunch(exec, e->stream) : enqueue();
         if (err != hipSuccess) return done(fail(VMC_E_DEVICE, std::string("optimize: ") + hipGetErrorString(err)));
     }
-    std::vector<double> hist((size_t)iterations * vmc::kOptOut);
     double agf[2];
     float ms = 0;
     if ((err = hipEventRecord(e->ev1, e->stream)) != hipSuccess ||
         (n > 0 && (err = hipMemcpyAsync(params, d_theta, sizeof(double) * n, hipMemcpyDeviceToHost,
                                         e->stream)) != hipSuccess) ||
         (err = hipMemcpyAsync(agf, d_ag, sizeof(agf), hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
-        (err = hipMemcpyAsync(hist.data(), d_hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost,
-                              e->stream)) != hipSuccess ||
+        (history && (err = hipMemcpyAsync(history, d_hist,
+                                          sizeof(double) * (size_t)iterations * vmc::kOptOut,
+                                          hipMemcpyDeviceToHost, e->stream)) != hipSuccess) ||
         (err = hipStreamSynchronize(e->stream)) != hipSuccess ||
         (err = hipEventElapsedTime(&ms, e->ev0, e->ev1)) != hipSuccess)
         return done(fail(VMC_E_DEVICE, std::string("optimize: ") + hipGetErrorString(err)));
@@ -737,9 +779,23 @@ int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int e
     e->gamma = agf[1];
     e->counter += (uint64_t)iterations * total;
     e->init_qf = false;
-    if (history) std::copy(hist.begin(), hist.end(), history);
     if (kernel_ms) *kernel_ms = ms;
     return done(VMC_OK);
+}
+
+int vmc_optimize(vmc_engine* e, double* params, int iterations, int steps, int equilibration,
+                 const vmc_optimizer_config* opt, int use_graph, double* history, double* kernel_ms)
+{
+    return optimize_impl(e, params, iterations, steps, equilibration, opt, use_graph, history,
+                         kernel_ms, false);
+}
+
+int vmc_optimize_static(vmc_engine* e, double* params, int iterations, int steps,
+                        int equilibration, const vmc_optimizer_config* opt, double* history,
+                        double* kernel_ms)
+{
+    return optimize_impl(e, params, iterations, steps, equilibration, opt, 0, history, kernel_ms,
+                         true);
 }
 
 int vmc_run(vmc_engine* e, int steps, int equilibration, int iteration, int batches,

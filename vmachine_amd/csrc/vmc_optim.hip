@@ -11,6 +11,8 @@
 // iteration's values from OptState, so one captured iteration replays.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "vmc_device.h"
 
 namespace vmc {
@@ -26,8 +28,12 @@ __global__ __launch_bounds__(256) void opt_step_kernel(const OptLayout L, const 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = L.rows * L.mp;
     double* h = hist + (size_t)(st->iteration - it0) * kOptOut;
-    if (i == 0)
+    if (i == 0) {
         for (int k = 0; k < kNumWalkerAcc; k++) h[k] = acc[k];
+        // every entry of the row is written: alpha / gamma are 0 without their element
+        if (L.row_gauss < 0) h[kOptAlpha] = 0.0;
+        if (L.row_pade < 0) h[kOptGamma] = 0.0;
+    }
     if (i >= n) return;
     const int r = i / L.mp, c = i - r * L.mp;
     const int half = L.nd + L.H + L.nd * L.H;   // RBM rows: [a | b | W] sums, then E x the same
@@ -96,10 +102,53 @@ hipError_t launch_opt_tail(const OptLayout& L, const double* acc, const double* 
                            hipStream_t stream)
 {
     const int n = L.rows * L.mp;
-    hipLaunchKernelGGL(opt_step_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, L, acc, rbm,
+    // one block at least: thread 0 writes the iteration's history row
+    hipLaunchKernelGGL(opt_step_kernel, dim3(n > 0 ? (n + 255) / 256 : 1), dim3(256), 0, stream, L, acc, rbm,
                        theta, m, v, st, ag, rbm_a, rbm_b, rbm_W, hist, it0);
     hipLaunchKernelGGL(opt_advance_kernel, dim3(1), dim3(64), 0, stream, st, steps_total);
     return hipGetLastError();
+}
+
+// ---- device state of the allocation-free loop ----
+// One copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel: it uses the device address hipGetSymbolAddress
+// returns for the current device.
+static __device__ __align__(256) double scratch[kStaticDoubles];
+
+__global__ void fill_kernel(double* p, int n, double value)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+hipError_t launch_fill(double* p, int n, double value, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, n, value);
+    return hipGetLastError();
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+hipError_t static_scratch(double** out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return err;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[dev] = static_cast<double*>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
 }
 
 }  // namespace vmc

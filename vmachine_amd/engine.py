@@ -134,6 +134,8 @@ def lib():
         L.vmc_get_density.argtypes = [ctypes.c_void_p, _I64P, _I64P, _I64P]
         L.vmc_optimize.argtypes = [ctypes.c_void_p, _DP, _I, _I, _I,
                                    ctypes.POINTER(VmcOptimizerConfig), _I, _DP, _DP]
+        L.vmc_optimize_static.argtypes = [ctypes.c_void_p, _DP, _I, _I, _I,
+                                          ctypes.POINTER(VmcOptimizerConfig), _DP, _DP]
         L.vmc_batch_create.argtypes = [ctypes.POINTER(VmcBatchConfig),
                                        ctypes.POINTER(ctypes.c_void_p)]
         L.vmc_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -278,6 +280,20 @@ class Engine:
         self._check(self.L.vmc_optimize(self.h, _dp(p), iterations, steps, equilibration,
                                         ctypes.byref(oc), int(bool(use_graph)), _dp(hist),
                                         ctypes.byref(ms)))
+        return p, hist, ms.value
+
+    def optimize_static(self, params, iterations, steps, equilibration=0, kind="adam",
+                        learning_rate=0.1, gamma=0.0, monotonic=0.0, batches=None):
+        """As optimize(use_graph=False) through vmc_optimize_static, which
+        allocates no device memory (its loop state lives in a per-GPU array)."""
+        k = {"adam": OPT_ADAM, "sgd": OPT_SGD, "gd": OPT_GD}[kind]
+        oc = VmcOptimizerConfig(k, learning_rate, gamma, monotonic,
+                                batches if batches else (10 if k == OPT_ADAM else 1))
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(self.E, self.maxpar).copy()
+        hist = np.zeros((iterations, OPT_OUT))
+        ms = ctypes.c_double(0)
+        self._check(self.L.vmc_optimize_static(self.h, _dp(p), iterations, steps, equilibration,
+                                               ctypes.byref(oc), _dp(hist), ctypes.byref(ms)))
         return p, hist, ms.value
 
     def run(self, steps, equilibration=0, iteration=0, batches=1, record_walkers=0):
