@@ -96,21 +96,66 @@ def timed(eng, iters, warmup, spi):
     return wall, kern / 1000.0, E / n, acc / n
 
 
-def measure_iat(eng, steps, rec_walkers):
+def measure_iat(eng, steps, block, chunk=1 << 14):
+    """tau_int of E_L of the engine's walkers (all of them recorded) over
+    `steps` further steps, recorded in chunks and reduced to means of `block`
+    steps on the host.  tau_int_sokal: Sokal's windowed sum over the
+    autocorrelation about the mean pooled over all walkers (stats.iat_pooled),
+    which needs series only a few windows long; tau_int_blocking: per-walker
+    automated blocking, which reads low until each series is some hundred tau
+    long.  The IAT-corrected rates use the larger of the two (tau_int)."""
     from vmachine_amd import stats
-    _, _, _, en = eng.run(steps, 0, record_walkers=rec_walkers)
-    en = en.astype(np.float64)
-    tau_b = stats.iat_many(en, "blocking")
-    tau_s = stats.iat_many(en[: min(64, rec_walkers)], "sokal")
-    means = en.mean(axis=1)
-    return tau_b, tau_s, float(means.mean()), float(means.std() / np.sqrt(len(means)))
+    W = eng.cfg.num_walkers
+    Ys, s1, s2, cnt = [], 0.0, 0.0, 0
+    for _ in range(max(1, steps // chunk)):
+        _, _, _, en = eng.run(chunk, 0, record_walkers=W)
+        e = en.astype(np.float64)
+        s1 += e.sum()
+        s2 += (e * e).sum()
+        cnt += e.size
+        Ys.append(stats.block_means(e, block))
+    Y = np.concatenate(Ys, axis=1)
+    mean = s1 / cnt
+    var_step = s2 / cnt - mean * mean
+    r = stats.iat_pooled(Y, block, var_step)
+    tau = max(r["tau_int_sokal"], r["tau_int_blocking"])
+    r.update({"tau_int": tau, "series_steps": int(Y.shape[1] * block), "series_walkers": W,
+              "block_steps": block, "series_length_in_tau": Y.shape[1] * block / tau,
+              "energy": float(mean),
+              "energy_err": float(np.sqrt(2.0 * tau * var_step / cnt))})
+    return r
+
+
+def iat_engine(V, args, make, pos, stream):
+    """A fresh engine of --iat-walkers walkers from the first of the
+    equilibrated positions, with a random stream of its own."""
+    eng = make(args.iat_walkers)
+    eng.set_positions(pos[: args.iat_walkers])
+    eng.set_random_state(args.seed + stream, 0)
+    return eng
+
+
+def cpu_description():
+    """Host CPU model and core counts (the cores this process may use, and all)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, usable or 1, os.cpu_count() or 1
 
 
 def cpu_baseline(args):
     """The reference binary (oracle/_ref/vmachine, built from the reference's own
     sources) on the host cores, one MPI rank per core, on a bounded sample of the
     same workload; falls back to the oracle's multithreaded restatement."""
-    cores = min(16, os.cpu_count() or 1)
+    model, cores, total_cores = cpu_description()
+    host = {"cpu_model": model, "cores_usable": cores, "cores_total": total_cores}
     ref = os.path.join(ROOT, "oracle", "_ref", "vmachine")
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
     if os.path.exists(ref) and os.path.exists(mpiexec) and not args.cpu_port:
@@ -119,7 +164,7 @@ def cpu_baseline(args):
                 f"omega: {args.omega}\ninteractionStyle: coulomb\n"
                 "waveFunctionElement: slaterDeterminant\nwaveFunctionElement: gaussian\n"
                 "waveFunctionElement: padeJastrow\nbasis: hermite\n"
-                f"numIterations: 2\nnumSteps: {per_rank * cores}\nlearningRate: 0.0\n"
+                f"numIterations: 3\nnumSteps: {per_rank * cores}\nlearningRate: 0.0\n"
                 f"stepLength: {args.dt}\nsampling: importanceSampling\n")
         with tempfile.TemporaryDirectory() as td:
             with open(os.path.join(td, "input.in"), "w") as f:
@@ -132,17 +177,23 @@ def cpu_baseline(args):
                 wall = time.time() - t0
                 rows = [ln.split() for ln in r.stdout.splitlines()
                         if ln[:1].isdigit() and len(ln.split()) == 11]
-                if r.returncode == 0 and rows:
-                    # last column: sampling time summed over ranks for one iteration
-                    t_sum = float(rows[0][-1])
+                if r.returncode == 0 and len(rows) >= 2:
+                    # last column: sampling time summed over ranks for one
+                    # iteration.  The chains start from N(0, 1/omega) and keep
+                    # their positions between iterations: the first iteration
+                    # (the transient from that start) is discarded and the last
+                    # one timed.
+                    t_sum = float(rows[-1][-1])
                     steps = per_rank * cores + int(per_rank * cores * 0.001) * cores
                     rate = steps / (t_sum / cores)
-                    return {"value": rate, "unit": "walker-steps/s", "cores": cores,
-                            "kind": "reference",
-                            "sample": f"reference vmachine (FP64, importance sampling), "
-                                      f"mpiexec -n {cores}, {per_rank} steps/rank, one "
-                                      f"iteration timed by its own MPI_Wtime (wall {wall:.1f} s "
-                                      f"incl. start-up)"}
+                    out = {"value": rate, "unit": "walker-steps/s", "cores": cores,
+                           "kind": "reference",
+                           "sample": f"reference vmachine (FP64, importance sampling), "
+                                     f"mpiexec -n {cores}, {per_rank} steps/rank per iteration, "
+                                     f"3 iterations, the last timed by its own MPI_Wtime (wall "
+                                     f"{wall:.1f} s incl. start-up and the discarded iterations)"}
+                    out.update(host)
+                    return out
             except (OSError, subprocess.SubprocessError) as ex:
                 print(f"# reference baseline failed: {ex}", file=sys.stderr)
     from oracle import oracle as O
@@ -156,8 +207,10 @@ def cpu_baseline(args):
     t0 = time.perf_counter()
     O.run_many(cfg, P, pos, args.seed, counters, steps, 0, threads=cores)
     dt = time.perf_counter() - t0
-    return {"value": W * steps / dt, "unit": "walker-steps/s", "cores": cores, "kind": "port",
-            "sample": f"oracle restatement (FP64), {W} walkers x {steps} steps on {cores} threads"}
+    out = {"value": W * steps / dt, "unit": "walker-steps/s", "cores": cores, "kind": "port",
+           "sample": f"oracle restatement (FP64), {W} walkers x {steps} steps on {cores} threads"}
+    out.update(host)
+    return out
 
 
 def rbmpj_workload(args, V):
@@ -393,14 +446,15 @@ def equilibrate(V, args):
         return eng.positions()
 
 
-def lp_engine(V, args, pos):
+def lp_engine(V, args, pos, walkers=None):
+    W = walkers or args.walkers
     cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
                         sampling=V.LEARNED, step_length=args.dt, quirks=0, precision=32,
-                        refresh_interval=args.refresh, num_walkers=args.walkers)
+                        refresh_interval=args.refresh, num_walkers=W)
     eng = V.Engine(cfg)
     eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
     eng.lp_set(lp_initial_weights(args.seed))
-    eng.set_positions(pos)
+    eng.set_positions(pos[:W])
     eng.set_random_state(args.seed + 14, 0)
     return eng
 
@@ -456,9 +510,14 @@ def main():
     ap.add_argument("--equil-steps", type=int, default=60000)
     ap.add_argument("--equil-dt", type=float, default=0.1)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--iat-steps", type=int, default=32768)
-    ap.add_argument("--iat-walkers", type=int, default=256)
-    ap.add_argument("--lp-iat-steps", type=int, default=8192)
+    ap.add_argument("--iat-steps", type=int, default=1 << 20,
+                    help="recorded steps per walker for tau_int of the Langevin forms")
+    ap.add_argument("--iat-walkers", type=int, default=2048,
+                    help="walkers of the tau_int runs (every one recorded)")
+    ap.add_argument("--iat-block", type=int, default=64,
+                    help="steps per block mean of the recorded Langevin series")
+    ap.add_argument("--lp-iat-steps", type=int, default=1 << 16)
+    ap.add_argument("--lp-iat-block", type=int, default=4)
     ap.add_argument("--lp-train-iters", type=int, default=300)
     ap.add_argument("--lp-train-steps", type=int, default=10)
     ap.add_argument("--lp-lr", type=float, default=3e-3)
@@ -469,6 +528,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-iat", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the mixed-size batch figure of the default workload")
+    ap.add_argument("--batch-main", default="n2:131072",
+                    help="walker allotment of the batch figure in the default workload")
     ap.add_argument("--workload", choices=["dot56", "rbmpj", "batch", "copy"], default="dot56",
                     help="dot56: the headline N=56 workload; rbmpj: RBM x Slater x Pade N=12 H=24; "
                          "batch: seven dots N=2..56 walked side by side; "
@@ -527,12 +590,12 @@ def main():
     F = flops_per_step(args.N, n, a=acc32)
     B4 = bytes_per_step(args.N, n, 4, a=acc32)
     achieved = F * steps_per_launch / launch_s / 1e12
+    eng.close()
     iat = None
     if not args.no_iat:
-        tb, ts, Em, Ee = measure_iat(eng, args.iat_steps, args.iat_walkers)
-        iat = {"tau_int_blocking": tb, "tau_int_sokal": ts, "energy": Em, "energy_err": Ee,
-               "series_steps": args.iat_steps, "series_walkers": args.iat_walkers}
-    eng.close()
+        with iat_engine(V, args, lambda w: make_engine(V, args, 32, w), pos, 21) as e:
+            iat = measure_iat(e, args.iat_steps, args.iat_block)
+        print(f"# FP32 tau_int {iat['tau_int']:.0f} steps", file=sys.stderr, flush=True)
 
     # ---- plain FP64 walk, exact Langevin dt ----
     fp64 = None
@@ -545,11 +608,13 @@ def main():
         fp64 = {"walker_steps_per_s": r64, "energy": E64, "acceptance": a64,
                 "kernel_walker_steps_per_s": args.walkers * args.steps_per_iter *
                 args.fp64_iters / k64}
-        if not args.no_iat:
-            tb, ts, Em, Ee = measure_iat(eng, args.iat_steps, args.iat_walkers)
-            fp64.update({"tau_int_blocking": tb, "energy_iat_run": Em, "energy_err": Ee,
-                         "iat_corrected_samples_per_s": r64 / (2 * tb)})
         eng.close()
+        if not args.no_iat:
+            with iat_engine(V, args, lambda w: make_engine(V, args, 64, w), pos, 22) as e:
+                it64 = measure_iat(e, args.iat_steps, args.iat_block)
+            fp64["iat"] = it64
+            fp64["iat_corrected_samples_per_s"] = r64 / (2 * it64["tau_int"])
+            print(f"# FP64 tau_int {it64['tau_int']:.0f} steps", file=sys.stderr, flush=True)
 
     # ---- learned proposal, FP32: online training, then the same measurements ----
     learned = None
@@ -568,32 +633,54 @@ def main():
                                 "esjd_start": float(hist[:5, 0].mean()),
                                 "esjd_end": float(hist[-5:, 0].mean()),
                                 "acceptance_end": float(hist[-5:, 1].mean())}}
-        if not args.no_iat:
-            tb, ts, Em, Ee = measure_iat(eng, args.lp_iat_steps, args.iat_walkers)
-            learned.update({"tau_int_blocking": tb, "tau_int_sokal": ts, "energy_iat_run": Em,
-                            "energy_err": Ee, "iat_corrected_samples_per_s": rl / (2 * tb)})
+        theta = eng.lp_get()
         eng.close()
+        if not args.no_iat:
+            def make_lp(w):
+                e = lp_engine(V, args, pos, walkers=w)
+                e.lp_set(theta)
+                return e
+            with iat_engine(V, args, make_lp, pos, 23) as e:
+                itl = measure_iat(e, args.lp_iat_steps, args.lp_iat_block)
+            learned["iat"] = itl
+            learned["iat_corrected_samples_per_s"] = rl / (2 * itl["tau_int"])
+            print(f"# learned tau_int {itl['tau_int']:.1f} steps", file=sys.stderr, flush=True)
 
     # ---- the reference's own method (Green-function ratio, quirks) for the CPU's tau ----
     tau_ref = None
     if not args.no_iat and not args.no_cpu:
-        cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
-                            sampling=V.IMPORTANCE, step_length=args.dt, precision=32,
-                            refresh_interval=args.refresh, num_walkers=args.walkers)
-        with V.Engine(cfg) as eng:
-            eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
-            eng.set_positions(pos)
-            eng.set_random_state(args.seed + 13, 0)
+        def make_compat(w):
+            cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
+                                sampling=V.IMPORTANCE, step_length=args.dt, precision=32,
+                                refresh_interval=args.refresh, num_walkers=w)
+            e = V.Engine(cfg)
+            e.set_parameters(np.array([[1.0], [1.0], [1.0]]))
+            return e
+        with iat_engine(V, args, make_compat, pos, 13) as eng:
             eng.run(5000, 0)
-            tau_ref = measure_iat(eng, args.iat_steps, args.iat_walkers)[0]
+            tau_ref = measure_iat(eng, args.iat_steps, args.iat_block)
 
     cpu = None if args.no_cpu else cpu_baseline(args)
     if cpu and tau_ref:
-        cpu["tau_int_blocking"] = tau_ref
-        cpu["iat_corrected_samples_per_s"] = cpu["value"] / (2 * tau_ref)
+        cpu["iat"] = tau_ref
+        cpu["iat_corrected_samples_per_s"] = cpu["value"] / (2 * tau_ref["tau_int"])
         cpu["tau_note"] = ("tau of E_L for the reference's own method (its Green-function ratio "
                            "and Pade ratio semantics) measured with the engine's compatible mode "
                            "on the same equilibrated walkers")
+
+    # ---- the mixed-size batch (seven dots, walkers in proportion to N^2) ----
+    batch = None
+    if not args.no_batch:
+        bl = batch_measure(args, V, batch_allotment(args.batch_main))
+        batch = {"walker_steps_per_s": bl["value"], "allotment": args.batch_main,
+                 "walkers_per_system": bl["config"]["walkers_per_system"],
+                 "forms": bl["forms"], "autotune": bl["autotune"],
+                 "speedup_vs_sequential_engines": bl["speedup_vs_sequential_engines"],
+                 "energies_last_iteration": bl["energies_last_iteration"],
+                 "note": "one optimisation iteration = 200 steps per walker of every system + "
+                         "device ADAM; graph = side-by-side kernels replayed as a hipGraph; "
+                         "sequential = seven single-system engines in turn with the host "
+                         "optimiser"}
 
     out = {
         "metric": "walker-steps/s and IAT-corrected samples/s at N=56 w=0.1, vs plain FP64",
@@ -613,25 +700,51 @@ def main():
                                f"exact MH Langevin dt={args.dt}, drift limit a={args.drift_limit}",
                    "walkers": args.walkers, "steps_per_iter": args.steps_per_iter,
                    "refresh_interval": args.refresh, "equilibration_steps": args.equil_steps,
-                   "equilibration_dt": args.equil_dt},
+                   "equilibration_dt": args.equil_dt,
+                   "langevin": ("exact Metropolis-Hastings Langevin (the Green's-function ratio "
+                                "in the acceptance) with the Umrigar-Nightingale-Runge drift "
+                                f"limit a={args.drift_limit}: F -> 2F/(1+sqrt(1+a F^2 dt/2)) in "
+                                "the proposal and its density; still exact, and walkers next "
+                                "to a node of the determinant are not frozen by its divergent "
+                                "drift" if args.drift_limit > 0 else
+                                "exact Metropolis-Hastings Langevin, no drift limit"),
+                   "refresh": (f"FP64 Slater-inverse rebuild every {args.refresh} steps of "
+                               "every walker (the FP32 Langevin form accepts 0.9999 of its "
+                               "moves, so that is 100 accepted moves; the learned form, which "
+                               f"accepts about 0.23, rebuilds every {args.refresh} steps too, "
+                               "about 23 accepted moves)")},
         "kernel_walker_steps_per_s": kern_rate,
         "energy": E32,
         "acceptance": acc32,
         "iat": iat,
-        "iat_corrected_samples_per_s": (value / (2 * iat["tau_int_blocking"])) if iat else None,
+        "iat_note": (f"tau_int of E_L per form from {args.iat_walkers} walkers x {args.iat_steps} "
+                     f"steps (Langevin forms; learned: x {args.lp_iat_steps}) on fresh engines "
+                     "started from the same equilibrated positions: Sokal's windowed sum over "
+                     "the autocorrelation about the mean pooled over all walkers, and per-walker "
+                     "automated blocking; tau_int = the larger; IAT-corrected samples/s = "
+                     "walker-steps/s / (2 tau_int)"),
+        "iat_corrected_samples_per_s": (value / (2 * iat["tau_int"])) if iat else None,
         "fp64": fp64,
         "learned": learned,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": PMC_BYTES_PER_WALKER_LAUNCH * args.walkers,
-                     "note": "VALU FP32 compute roofline (no MFMA in the walk kernel; walker "
-                             "state is register-resident); algorithmic flops "
+                     "peak_nonpacked": PEAK_FP32_TFLOPS / 2,
+                     "frac_nonpacked": achieved / (PEAK_FP32_TFLOPS / 2),
+                     "note": "VALU FP32 compute roofline: the walk kernel issues no MFMA and "
+                             "keeps walker state in registers, so HBM is far from its bound. "
+                             "peak = MI355X's 157.3 TFLOP/s vector FP32, which counts packed "
+                             "FP32 (v_pk_fma_f32); the build keeps packed FP32 out of every "
+                             "code object, so the reachable VALU ceiling is half of it "
+                             "(peak_nonpacked, frac_nonpacked).  Algorithmic flops "
                              f"{F:.0f}/walker-step x {steps_per_launch} walker-steps per launch; "
                              f"streaming-equivalent bytes {B4:.0f}/walker-step; traffic = HBM "
                              "bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE "
                              "(scaled by a 1 GiB copy) at N=56, 100 steps per launch"},
         "cpu_baseline": cpu,
     }
+    if batch:
+        out["batch_c4"] = batch
     if cpu:
         out["gpu_vs_cpu"] = value / cpu["value"]
     if learned and fp64 and "iat_corrected_samples_per_s" in learned and \

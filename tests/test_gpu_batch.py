@@ -26,6 +26,8 @@ pytestmark = pytest.mark.gpu
 KINDS_SGP = [O.SLATER, O.GAUSSIAN, O.PADE]
 # (N, walkers, dt): deliberately not in cost order, ragged walker counts
 SYSTEMS = [(12, 37, 0.02), (2, 100, 0.1), (6, 64, 0.05), (20, 45, 0.01)]
+# the three largest closed shells, few walkers each (the oracle is the slow side)
+SYSTEMS_LARGE = [(42, 19, 0.005), (30, 24, 0.01), (56, 13, 0.005)]
 LANES = {1: 1, 3: 4, 6: 8, 10: 16, 15: 16, 21: 32, 28: 32}   # orbitals per spin -> default K
 ORBITALS = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28}
 
@@ -75,18 +77,20 @@ def _check_sums(out_s, sums, g, ge, rtol):
     assert out_s[13] == sums["nonfinite"]
 
 
+@pytest.mark.parametrize("large", [False, True])
 @pytest.mark.parametrize("sampling,drift", [(O.IMPORTANCE_COMPAT, 0.0),
                                             (O.IMPORTANCE_EXACT, 1.0),
                                             (O.BRUTE_FORCE, 0.0)])
-def test_batch_retraces_the_oracle(sampling, drift):
+def test_batch_retraces_the_oracle(sampling, drift, large):
     V = _require_gpu()
     rng = np.random.default_rng(41 + sampling)
-    systems = _systems(rng)
+    systems = _systems(rng, SYSTEMS_LARGE if large else SYSTEMS)
     if sampling == O.BRUTE_FORCE:
         for y in systems:
             y["step_length"] = 0.5 / np.sqrt(y["omega"])
     steps, equil, seed = 120, 10, 500 + sampling
-    print(f"batch vs oracle sampling={sampling}", flush=True)
+    print(f"batch vs oracle sampling={sampling} N={[y['num_particles'] for y in systems]}",
+          flush=True)
     pos0 = []
     with _batch(V, systems, sampling, drift=drift) as b:
         b.init_walkers(seed)   # the random key: system s draws from seed + s
@@ -252,3 +256,97 @@ def test_uneven_dots_zero_variance(precision):
         assert abs(E - E0) <= tol * E0, (y["num_particles"], E, E0)
         assert abs(E2 - E0 * E0) <= 4 * tol * E0 * E0
         assert out[1, s, 8] > 0.5 * M
+
+
+# The mixed-size batch configuration: seven closed-shell dots, walkers in
+# proportion to N^2 (131072 in total), FP32 walk with an FP64 inverse rebuild
+# every 100 steps, exact Langevin dt 0.005 with drift limit 1, autotuned
+# shapes, one optimisation iteration (device ADAM) per hipGraph replay.
+C4_DOTS = [(2, 1.0), (6, 1.0), (12, 1.0), (20, 0.5), (30, 0.5), (42, 0.28), (56, 0.1)]
+C4_TOTAL, C4_DT, C4_STEPS = 131072, 0.005, 200
+
+
+def _c4_systems():
+    w = np.array([N * N for N, _ in C4_DOTS], dtype=np.float64)
+    Ws = [max(1, int(round(C4_TOTAL * x / w.sum()))) for x in w]
+    return [dict(num_particles=N, omega=om, num_walkers=W, step_length=C4_DT, alpha=1.0,
+                 gamma=1.0) for (N, om), W in zip(C4_DOTS, Ws)]
+
+
+def _c4_batch(V, systems, precision):
+    return V.Batch(systems, kinds=KINDS_SGP, sampling=O.IMPORTANCE_EXACT, precision=precision,
+                   quirks=0, drift_limit=1.0, refresh_interval=100)
+
+
+def test_c4_mixed_batch_at_full_size():
+    """The C4 batch at full size.  (a) With the default shapes every system's
+    sums equal those of a single-system engine seeded seed + s (FP32: 1e-5
+    relative, >= 95 % of the walkers' positions within 1e-4).  (b) The FP64
+    batch from the same walkers and streams gives per-system energies within
+    4 standard errors of the FP32 batch's (the error of a mean over W walkers
+    is at most sigma / sqrt(W) whatever the autocorrelation within each
+    walker's 200 steps).  (c) After the autotuner, optimisation iterations
+    with ADAM replayed as a hipGraph equal direct launches bit for bit.  No
+    sampled local energy is non-finite."""
+    V = _require_gpu()
+    systems = _c4_systems()
+    Ws = [y["num_walkers"] for y in systems]
+    assert abs(sum(Ws) - C4_TOTAL) <= len(Ws)
+    seed = 8080
+    print(f"C4 batch: walkers {Ws}", flush=True)
+    res = {}
+    for prec in (32, 64):
+        with _c4_batch(V, systems, prec) as b:
+            b.init_walkers(seed)
+            out, ms = b.run(2, C4_STEPS, 0, use_graph=False)
+            res[prec] = (out, [b.get_positions(s) for s in range(len(systems))])
+        print(f"  precision {prec}: {ms:.1f} ms for 2 iterations", flush=True)
+    out32, pos32 = res[32]
+    out64, _ = res[64]
+    assert np.all(np.isfinite(out32)) and np.all(np.isfinite(out64))
+    for s, y in enumerate(systems):
+        N, W = y["num_particles"], y["num_walkers"]
+        M = W * C4_STEPS
+        assert out32[1, s, 13] == 0 and out64[1, s, 13] == 0
+        # (a) single-system engine, same seed + s
+        cfg = engine_config(N, y["omega"], KINDS_SGP, O.IMPORTANCE_EXACT, C4_DT, quirks=0,
+                            precision=32, walkers=W, refresh=100, drift_limit=1.0)
+        with V.Engine(cfg) as e:
+            e.set_parameters(_params(y))
+            e.init_walkers(seed + s)
+            r0 = e.run(C4_STEPS, 0, 0, 1)
+            r1 = e.run(C4_STEPS, 0, 1, 1)
+            epos = e.positions()
+        _check_sums(out32[0, s], *r0[:3], 1e-5)
+        _check_sums(out32[1, s], *r1[:3], 1e-5)
+        close = np.all(np.abs(pos32[s] - epos) < 1e-4, axis=1)
+        assert close.mean() >= 0.95, (N, close.mean())
+        # (b) FP32 against FP64
+        E32, E64 = out32[1, s, 0] / M, out64[1, s, 0] / M
+        v32 = out32[1, s, 1] / M - E32 * E32
+        v64 = out64[1, s, 1] / M - E64 * E64
+        se = np.sqrt((v32 + v64) / W)
+        print(f"  N={N} W={W}: E32 {E32:.6f} E64 {E64:.6f} |diff| {abs(E32 - E64):.2e} "
+              f"(4 se {4 * se:.2e}), acceptance {out32[1, s, 8] / M:.4f}", flush=True)
+        assert abs(E32 - E64) <= 4.0 * se + 1e-9 * abs(E64)
+        assert out32[1, s, 8] > 0.9 * M
+    # (c) autotuned shapes, ADAM, graph against direct launches
+    runs = []
+    lanes = None
+    for use_graph in (True, False):
+        with _c4_batch(V, systems, 32) as b:
+            b.init_walkers(seed)
+            if lanes is None:
+                lanes, ns = b.autotune(30)
+                print(f"  autotune lanes {lanes.tolist()}", flush=True)
+            else:
+                for s in range(len(systems)):
+                    b.set_shape(s, int(lanes[s]))
+            out, ms = b.run(3, C4_STEPS, 0, learning_rate=0.01, use_graph=use_graph)
+            runs.append((out, [b.get_positions(s) for s in range(len(systems))], b.parameters()))
+            print(f"  graph={use_graph}: {ms:.1f} ms for 3 iterations", flush=True)
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    for a, c in zip(runs[0][1], runs[1][1]):
+        np.testing.assert_array_equal(a, c)
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])
+    assert np.all(np.isfinite(runs[0][0])) and np.all(runs[0][0][:, :, 13] == 0)
