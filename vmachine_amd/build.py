@@ -23,8 +23,10 @@ CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
 ARCH = os.environ.get("VMC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-# packed FP32 instructions are kept out of every code object
-HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# packed FP32 instructions are kept out of every code object.  -fno-signed-zeros
+# lets the compiler drop additions of a zero (a sum started at 0, fma(a, b, 0));
+# the sign of a zero reaches no result of the engine.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fno-signed-zeros",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 

@@ -93,6 +93,14 @@ template <> struct M<float> {
         return __builtin_amdgcn_logf(x) * 0.693147180559945309417f;
     }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    // distance r and its inverse from r^2 with one reciprocal square root
+    // (v_rsq_f32) instead of a square root and a reciprocal; a masked pair
+    // (val false: the particle itself, padding lanes) gets r = 1/r = 0
+    static __device__ __forceinline__ void dist(float r2, bool val, float& r, float& ir)
+    {
+        ir = val ? __builtin_amdgcn_rsqf(r2) : 0.f;
+        r = r2 * ir;
+    }
     static __device__ __forceinline__ float exp(float x) { return __expf(x); }
     static __device__ __forceinline__ float log(float x) { return __logf(x); }
     // v_cos_f32 takes its argument in revolutions: cos(2 pi u)
@@ -103,6 +111,11 @@ template <> struct M<double> {
     static __device__ __forceinline__ double fsqrt(double x) { return ::sqrt(x); }
     static __device__ __forceinline__ double flog(double x) { return ::log(x); }
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ void dist(double r2, bool val, double& r, double& ir)
+    {
+        r = ::sqrt(r2);
+        ir = val ? 1.0 / r : 0.0;
+    }
     static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
     static __device__ __forceinline__ double log(double x) { return ::log(x); }
     static __device__ __forceinline__ double cos2pi(double u) { return ::cos(6.283185307179586476925286766559 * u); }
@@ -317,6 +330,7 @@ struct Walker {
     bool upspin[2];
     int l, gbase, g;
     double th_alpha, th_gamma;   // Gaussian alpha and Pade gamma of the walk
+    R pa_same, pa_anti;          // Pade a of a spin-parallel / antiparallel pair
 
     // RBM state (RB only).  Lane l owns hidden units h = l + K j; the sorted
     // coordinate vector of the walker and its n = sigmoid(v) live in LDS.
@@ -351,6 +365,16 @@ struct Walker {
         R hx[S + 1], hy[S + 1];
         hermite<R, S>(t0, hx);
         hermite<R, S>(t1, hy);
+        // first and second derivative rows h_n' = d1_n h_{n-1}, h_n'' = d2_n h_{n-2}
+        // (formed once, so each orbital term below is a single FMA)
+        R dx[S + 1], ddx[S + 1], dy[S + 1], ddy[S + 1];
+#pragma unroll
+        for (int i = 0; i <= S; i++) {
+            dx[i] = i >= 1 ? R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0] : R(0);
+            dy[i] = i >= 1 ? R(kHC.d1[i]) * hy[i >= 1 ? i - 1 : 0] : R(0);
+            ddx[i] = i >= 2 ? R(kHC.d2[i]) * hx[i >= 2 ? i - 2 : 0] : R(0);
+            ddy[i] = i >= 2 ? R(kHC.d2[i]) * hy[i >= 2 ? i - 2 : 0] : R(0);
+        }
         R u0[S + 1], u1[S + 1], u2[S + 1];
 #pragma unroll
         for (int i = 0; i <= S; i++) u0[i] = u1[i] = u2[i] = R(0);
@@ -359,14 +383,14 @@ struct Walker {
             const int nx = orb_nx(k), ny = orb_ny(k);
             const R B = col[k];
             u0[nx] += B * hy[ny];
-            if (ny >= 1) u1[nx] += B * R(kHC.d1[ny]) * hy[ny >= 1 ? ny - 1 : 0];
-            if (ny >= 2) u2[nx] += B * R(kHC.d2[ny]) * hy[ny >= 2 ? ny - 2 : 0];
+            if (ny >= 1) u1[nx] += B * dy[ny];
+            if (ny >= 2) u2[nx] += B * ddy[ny];
         }
         R ax = R(0), ay = R(0), ss = R(0);
 #pragma unroll
         for (int i = 0; i <= S; i++) {
-            if (i >= 1) ax += R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0] * u0[i];
-            if (i >= 2) ss += R(kHC.d2[i]) * hx[i >= 2 ? i - 2 : 0] * u0[i];
+            if (i >= 1) ax += dx[i] * u0[i];
+            if (i >= 2) ss += ddx[i] * u0[i];
             ay += hx[i] * u1[i];
             ss += hx[i] * u2[i];
         }
@@ -880,7 +904,7 @@ struct ParticleMove {
 
 template <typename R, int NP, int K, bool RB>
 __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkParams& P, int k,
-                                            R z, R um, R ua, bool forced, R xforced,
+                                            R z, R um, R ua, R lua, bool forced, R xforced,
                                             StepInfo<R>& info,
                                             const ParticleMove<R>* pm = nullptr)
 {
@@ -941,11 +965,10 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         const bool val = w.valid[s] && q != p;
         const R e0 = w.x[s][0] - xp0, e1 = w.x[s][1] - xp1;
         const R n0 = w.x[s][0] - xn0, n1 = w.x[s][1] - xn1;
-        const R ro = M<R>::fsqrt(e0 * e0 + e1 * e1);
-        const R rn = M<R>::fsqrt(n0 * n0 + n1 * n1);
-        const R iro = val ? M<R>::rcp(ro) : R(0);
-        const R irn = val ? M<R>::rcp(rn) : R(0);
-        const R a = (w.upspin[s] == pup) ? R(P.a_same) : R(P.a_anti);
+        R ro, rn, iro, irn;
+        M<R>::dist(e0 * e0 + e1 * e1, val, ro, iro);
+        M<R>::dist(n0 * n0 + n1 * n1, val, rn, irn);
+        const R a = (w.upspin[s] == pup) ? w.pa_same : w.pa_anti;
         const R fo = M<R>::rcp(R(1) + gam * ro), fn = M<R>::rcp(R(1) + gam * rn);
         const R ho = ro * fo, hn = rn * fn;
         const R uo = a * fo * fo, un = a * fn * fn;
@@ -1000,6 +1023,10 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         // phi(r').col = sum_nx h_nx(x') u[nx], u[nx] = sum_ny h_ny(y') col(nx, ny); the
         // derivative rows reuse u (d/dx') and v[ny] = sum_nx h_nx(x') col (d/dy')
         const bool exact = P.sampling == kImportanceExact;
+        // v[nx] = sum_ny h_ny'(y') col(nx, ny) (the d/dy' row, exact Langevin only)
+        R dy[WK::S + 1];
+#pragma unroll
+        for (int i = 0; i <= WK::S; i++) dy[i] = i >= 1 ? R(kHC.d1[i]) * hy[i >= 1 ? i - 1 : 0] : R(0);
         R u[WK::S + 1], v[WK::S + 1];
 #pragma unroll
         for (int i = 0; i <= WK::S; i++) u[i] = v[i] = R(0);
@@ -1007,15 +1034,15 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         for (int kk = 0; kk < NP; kk++) {
             const int nx = orb_nx(kk), ny = orb_ny(kk);
             u[nx] += hy[ny] * w.a0[kk];
-            if (exact) v[ny] += hx[nx] * w.a0[kk];
+            if (exact && ny >= 1) v[nx] += dy[ny] * w.a0[kk];
         }
         R Tx = R(0), Ty = R(0);
 #pragma unroll
         for (int i = 0; i <= WK::S; i++) {
             Sl += hx[i] * u[i];
-            if (exact && i >= 1) {
-                Tx += R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0] * u[i];
-                Ty += R(kHC.d1[i]) * hy[i >= 1 ? i - 1 : 0] * v[i];
+            if (exact) {
+                if (i >= 1) Tx += (R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0]) * u[i];
+                Ty += hx[i] * v[i];
             }
         }
         Rs = __shfl(Sl, osrc, 64);
@@ -1124,7 +1151,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
     if (forced) {
         accept = true;
     } else if (pm) {
-        accept = log_ratio() + pm->logq >= M<R>::flog(ua);
+        accept = log_ratio() + pm->logq >= lua;
     } else if (P.sampling == kBruteForce) {
         accept = !(ratio < ua);
     } else if (P.sampling == kImportanceCompat) {
@@ -1137,7 +1164,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
             // FP32: the same test in the log domain, log w = softplus(xg) + log ratio,
             // which neither overflows nor forms inf x 0 next to a node
             const R sp = (xg > R(0) ? xg : R(0)) + M<R>::flog(R(1) + M<R>::exp(-(xg > R(0) ? xg : -xg)));
-            accept = sp + log_ratio() >= M<R>::flog(ua);
+            accept = sp + log_ratio() >= lua;
         }
     } else {
         R Gn = R(0);
@@ -1157,7 +1184,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
         if constexpr (std::is_same<R, double>::value)
             accept = ratio * M<R>::exp(lg) >= ua;
         else
-            accept = log_ratio() + lg >= M<R>::flog(ua);
+            accept = log_ratio() + lg >= lua;
     }
     if (!accept) return false;
 
@@ -1888,6 +1915,8 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     WK w;
     w.th_alpha = D.alpha;
     w.th_gamma = D.gamma;
+    w.pa_same = R(P.a_same);
+    w.pa_anti = R(P.a_anti);
     const int lane = threadIdx.x;
     w.l = lane & (K - 1);
     w.gbase = lane & ~(K - 1);
@@ -1952,18 +1981,23 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
 
     // random draws for K steps at a time, one step per lane of the group
     int rk = 0;
-    R rz = R(0), rum = R(0), rua = R(0), rz2 = R(0);
+    R rz = R(0), rum = R(0), rua = R(0), rz2 = R(0), rlua = R(0);
+    // the FP32 walks and the learned moves test acceptance in the log domain
+    constexpr bool log_accept = sizeof(R) == 4 || LP;
 
     for (int t = D.t_begin; t < D.t_end; t++) {
         const int tk = t & (K - 1);
-        if (tk == 0 || t == D.t_begin)
+        if (tk == 0 || t == D.t_begin) {
             step_draws<R>(P.seed, (uint32_t)walker, D.counter0 + (uint64_t)(t - tk + w.l), nd, rk,
                           rz, rum, rua, rz2);
+            if constexpr (log_accept) rlua = M<R>::flog(rua);
+        }
         const int src = w.gbase + tk;
         const int k = __shfl(rk, src, 64);
         const R z = __shfl(rz, src, 64);
         const R um = __shfl(rum, src, 64);
         const R ua = __shfl(rua, src, 64);
+        const R lua = log_accept ? __shfl(rlua, src, 64) : R(0);
 
         StepInfo<R> info;
         bool accept;
@@ -1971,9 +2005,9 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
             // electron k/2, normals (z, z2) of the same Box-Muller pair
             const R z2 = __shfl(rz2, src, 64);
             const ParticleMove<R> mv = lp_propose(w, P, LL, k >> 1, z, z2);
-            accept = walker_step(w, P, k, z, um, ua, false, R(0), info, &mv);
+            accept = walker_step(w, P, k, z, um, ua, lua, false, R(0), info, &mv);
         } else {
-            accept = walker_step(w, P, k, z, um, ua, false, R(0), info);
+            accept = walker_step(w, P, k, z, um, ua, lua, false, R(0), info);
         }
         if constexpr (RB) w.rbm_after_step(P, accept);
 
@@ -2237,6 +2271,8 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
     WK w;
     w.th_alpha = P.alpha;
     w.th_gamma = P.gamma;
+    w.pa_same = R(P.a_same);
+    w.pa_anti = R(P.a_anti);
     const int lane = threadIdx.x;
     w.l = lane & (K - 1);
     w.gbase = lane & ~(K - 1);
@@ -2264,9 +2300,9 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
             if (O.newv2) {
                 const ParticleMove<R> mv{moves[walker], R(O.newv2[2 * walker]),
                                          R(O.newv2[2 * walker + 1]), R(0)};
-                walker_step(w, P, 0, R(0), R(0), R(0), true, R(0), info, &mv);
+                walker_step(w, P, 0, R(0), R(0), R(0), R(0), true, R(0), info, &mv);
             } else {
-                walker_step(w, P, moves[walker], R(0), R(0), R(0), true, R(newv[walker]), info);
+                walker_step(w, P, moves[walker], R(0), R(0), R(0), R(0), true, R(newv[walker]), info);
             }
             if constexpr (RB) w.rbm_after_step(P, true);
         }
