@@ -4,8 +4,8 @@ parameter update, iteration after iteration without a return to the host.
 
 Pinned against the host loop the reference runs (ref src/system.cpp:81-139):
 vmc_run for one iteration's sums, the host optimiser (libvmc_host, whose ADAM,
-SGD and GD are pinned to the reference's own iterations in
-test_host_optimizer.py) for the update, vmc_set_parameters for the next
+SGD and GD are pinned to the reference's own iterations by
+test_optimizer_replays_reference_iterations in test_host.py) for the update, vmc_set_parameters for the next
 iteration.  FP64 walks on the same random streams: every iteration's sums and
 the final parameter matrix agree to 1e-9 relative (the device's pow and the
 host's may differ in the last bit, which the chains do not see).  A captured
