@@ -243,6 +243,26 @@ __device__ __forceinline__ T gprod(T v)
     return v;
 }
 
+// the values of lanes l and l ^ 16 (and of l and l ^ 32) by a VALU lane swap
+// between neighbouring 16-lane rows (v_permlane16_swap_b32; v_permlane32_swap_b32
+// between the wave's halves): swapping v with itself leaves, in every lane,
+// its own value in one result and its partner's in the other, in an order
+// that depends on the row, so only symmetric operations of the two may follow.
+// Lower latency than a lane shuffle through the LDS crossbar, for dependent
+// chains; all lanes must be active.
+__device__ __forceinline__ void xchg16(uint32_t v, uint32_t& a, uint32_t& b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    a = r[0];
+    b = r[1];
+}
+__device__ __forceinline__ void xchg32(uint32_t v, uint32_t& a, uint32_t& b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    a = r[0];
+    b = r[1];
+}
+
 template <int K>
 __device__ __forceinline__ uint32_t gmax32(uint32_t v)
 {
@@ -251,8 +271,15 @@ __device__ __forceinline__ uint32_t gmax32(uint32_t v)
     if constexpr (K >= 4) take((uint32_t)dpp<kDppXor2>((int)v));
     if constexpr (K >= 8) take((uint32_t)dpp<kDppHalfMirror>((int)v));
     if constexpr (K >= 16) take((uint32_t)dpp<kDppMirror>((int)v));
-    if constexpr (K >= 32) take(__shfl_xor(v, 16, 64));
-    if constexpr (K >= 64) take(__shfl_xor(v, 32, 64));
+    uint32_t a, b;
+    if constexpr (K >= 32) {
+        xchg16(v, a, b);
+        v = a > b ? a : b;
+    }
+    if constexpr (K >= 64) {
+        xchg32(v, a, b);
+        v = a > b ? a : b;
+    }
     return v;
 }
 
@@ -1364,8 +1391,15 @@ __device__ __forceinline__ uint32_t gmin32(uint32_t v)
     if constexpr (K >= 4) take((uint32_t)dpp<kDppXor2>((int)v));
     if constexpr (K >= 8) take((uint32_t)dpp<kDppHalfMirror>((int)v));
     if constexpr (K >= 16) take((uint32_t)dpp<kDppMirror>((int)v));
-    if constexpr (K >= 32) take(__shfl_xor(v, 16, 64));
-    if constexpr (K >= 64) take(__shfl_xor(v, 32, 64));
+    uint32_t a, b;
+    if constexpr (K >= 32) {
+        xchg16(v, a, b);
+        v = a < b ? a : b;
+    }
+    if constexpr (K >= 64) {
+        xchg32(v, a, b);
+        v = a < b ? a : b;
+    }
     return v;
 }
 
@@ -1534,7 +1568,8 @@ __device__ __forceinline__ lp_f4 lp_tile(uint4 a, uint4 b, lp_f4 c)
                                                    __builtin_bit_cast(lp_bf16x8, b), c, 0, 0, 0);
 }
 
-template <int K>
+// STAGED: the features are already in L.fs (lp_features_staged), f unused
+template <int K, bool STAGED = false>
 __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int l,
                                             const float (&f)[kLpIn], float (&o)[kLpOut])
 {
@@ -1545,7 +1580,7 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
     uint16_t* hb = reinterpret_cast<uint16_t*>(L.h);   // [G][64] bf16 activations
     const uint32_t* W32 = reinterpret_cast<const uint32_t*>(L.W);
     // the walker's features, bf16 pairs, zero-padded to 32
-    if (l == 0) {
+    if (!STAGED && l == 0) {
         uint32_t p[16];
 #pragma unroll
         for (int m = 0; m < 16; m++)
@@ -1602,6 +1637,42 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(tt[r]) + L.w(kLpB3 + r);
+}
+
+// FP32 walks on matrix-core tiles (lp_mf): the features of electron i at
+// (xi, yi) written straight into the walker's bf16 staging row L.fs (word 0 the
+// electron's coordinates, word 1 + m the m-th nearest electron's relative
+// coordinates, words 9..15 the tile's zero padding, cleared once per launch).
+// The same selection and the same FP32 values as lp_features, but the lane that
+// owns the selected electron writes its word, so no round moves coordinates
+// between lanes.
+template <int NP, int K, bool RB>
+__device__ __forceinline__ void lp_features_staged(const Walker<float, NP, K, RB>& w,
+                                                   const WalkParams& P, const LpLds<float, K>& L,
+                                                   int i, float xi, float yi)
+{
+    const float s = float(P.sqrt_omega);
+    uint32_t* fs = L.fs + w.g * 16;
+    if (w.l == 0) fs[0] = __builtin_bit_cast(uint32_t, lp_pack(s * xi, s * yi));
+    uint32_t key[2], word[2];
+#pragma unroll
+    for (int sl = 0; sl < 2; sl++) {
+        const int q = sl * NP + w.l;
+        const float e0 = w.x[sl][0] - xi, e1 = w.x[sl][1] - yi;
+        const float d2 = e0 * e0 + e1 * e1;
+        key[sl] = (w.valid[sl] && q != i) ? ((__float_as_uint(d2) & ~63u) | (uint32_t)q) : ~0u;
+        word[sl] = __builtin_bit_cast(uint32_t, lp_pack(s * e0, s * e1));
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const uint32_t best = gmin32<K>(key[0] < key[1] ? key[0] : key[1]);
+        const bool found = best != ~0u;
+        const bool own0 = found && key[0] == best, own1 = found && key[1] == best;
+        if (own0 || own1) fs[1 + m] = own0 ? word[0] : word[1];
+        else if (!found && w.l == 0) fs[1 + m] = 0u;
+        key[0] = own0 ? ~0u : key[0];
+        key[1] = own1 ? ~0u : key[1];
+    }
 }
 
 template <typename R, int K>
@@ -1708,6 +1779,20 @@ __device__ __forceinline__ ParticleMove<R> lp_propose(const Walker<R, NP, K, RB>
     const R xi = __shfl(vsel(slot != 0, w.x[1][0], w.x[0][0]), w.gbase + li, 64);
     const R yi = __shfl(vsel(slot != 0, w.x[1][1], w.x[0][1]), w.gbase + li, 64);
     R f[kLpIn], o[kLpOut], mu[2], sg[2], lt[2];
+    if constexpr (lp_mf(sizeof(R), K)) {
+        // features staged in LDS by the lanes that own them
+        lp_features_staged(w, P, L, i, xi, yi);
+        lp_mlp_mfma<K, true>(L, w.g, w.l, f, o);
+        lp_head(P, o, mu, sg, lt);
+        const R xn = xi + mu[0] + sg[0] * z0, yn = yi + mu[1] + sg[1] * z1;
+        const R lqf = -R(0.5) * (z0 * z0 + z1 * z1) - (lt[0] + lt[1]);
+        lp_features_staged(w, P, L, i, xn, yn);
+        lp_mlp_mfma<K, true>(L, w.g, w.l, f, o);
+        lp_head(P, o, mu, sg, lt);
+        const R r0 = (xi - xn - mu[0]) * M<R>::rcp(sg[0]), r1 = (yi - yn - mu[1]) * M<R>::rcp(sg[1]);
+        const R lqr = -R(0.5) * (r0 * r0 + r1 * r1) - (lt[0] + lt[1]);
+        return ParticleMove<R>{i, xn, yn, lqr - lqf};
+    }
     lp_features(w, P, i, xi, yi, f);
     lp_mlp(L, w.g, w.l, f, o);
     lp_head(P, o, mu, sg, lt);
@@ -1936,6 +2021,8 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     if constexpr (LP && !lp_static) {
         lp_carve(LL, rbm_dyn);
         for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
+        if constexpr (lp_mf(sizeof(R), K))   // the feature rows' zero padding
+            for (int i = lane; i < G * 16; i += 64) LL.fs[i] = 0u;
         __syncthreads();
     }
     // the RBM state; W in the rebuild's LDS after the rebuild where it fits
@@ -1961,6 +2048,8 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
         __syncthreads();   // the rebuild's reads of lds_T are done
         lp_carve(LL, lds_T);
         for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
+        if constexpr (lp_mf(sizeof(R), K))   // the feature rows' zero padding
+            for (int i = lane; i < G * 16; i += 64) LL.fs[i] = 0u;
         __syncthreads();
     }
     if (D.init_qf && D.t_begin == 0) {
