@@ -331,6 +331,27 @@ __device__ __forceinline__ mf_f4 mf_tile(const float (&a)[8], const float (&b)[8
 }
 
 // ------------------------------------------------------------------------
+// orbital rows of one particle: h_n(t) of both coordinates and their first
+// derivatives h_n' = d1_n h_{n-1} (dx[0] = dy[0] = 0), t = sqrt(omega) x
+// ------------------------------------------------------------------------
+template <typename R, int S>
+struct OrbRows {
+    R hx[S + 1], hy[S + 1], dx[S + 1], dy[S + 1];
+};
+
+template <typename R, int S>
+__device__ __forceinline__ void orb_rows(R t0, R t1, OrbRows<R, S>& o)
+{
+    hermite<R, S>(t0, o.hx);
+    hermite<R, S>(t1, o.hy);
+#pragma unroll
+    for (int i = 0; i <= S; i++) {
+        o.dx[i] = i >= 1 ? R(kHC.d1[i]) * o.hx[i >= 1 ? i - 1 : 0] : R(0);
+        o.dy[i] = i >= 1 ? R(kHC.d1[i]) * o.hy[i >= 1 ? i - 1 : 0] : R(0);
+    }
+}
+
+// ------------------------------------------------------------------------
 // walker state
 // ------------------------------------------------------------------------
 template <typename R, int NP, int K, bool RB>
@@ -382,25 +403,18 @@ struct Walker {
 
     __device__ __forceinline__ R sw(const WalkParams& P) const { return R(P.sqrt_omega); }
 
-    // Slater log-derivatives of a particle at (px, py) from its column `col` of
-    // the block inverse
-    __device__ __forceinline__ void slater_derivs(const WalkParams& P, R px, R py,
-                                                  const R (&col)[NP], R& gx, R& gy, R& sec) const
+    // Slater log-derivatives of a particle with orbital rows `o` from its
+    // column `col` of the block inverse
+    __device__ __forceinline__ void slater_derivs_rows(const WalkParams& P, const OrbRows<R, S>& o,
+                                                       const R (&col)[NP], R& gx, R& gy, R& sec) const
     {
         const R s = sw(P);
-        const R t0 = s * px, t1 = s * py;
-        R hx[S + 1], hy[S + 1];
-        hermite<R, S>(t0, hx);
-        hermite<R, S>(t1, hy);
-        // first and second derivative rows h_n' = d1_n h_{n-1}, h_n'' = d2_n h_{n-2}
-        // (formed once, so each orbital term below is a single FMA)
-        R dx[S + 1], ddx[S + 1], dy[S + 1], ddy[S + 1];
+        // second-derivative rows h_n'' = d2_n h_{n-2}
+        R ddx[S + 1], ddy[S + 1];
 #pragma unroll
         for (int i = 0; i <= S; i++) {
-            dx[i] = i >= 1 ? R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0] : R(0);
-            dy[i] = i >= 1 ? R(kHC.d1[i]) * hy[i >= 1 ? i - 1 : 0] : R(0);
-            ddx[i] = i >= 2 ? R(kHC.d2[i]) * hx[i >= 2 ? i - 2 : 0] : R(0);
-            ddy[i] = i >= 2 ? R(kHC.d2[i]) * hy[i >= 2 ? i - 2 : 0] : R(0);
+            ddx[i] = i >= 2 ? R(kHC.d2[i]) * o.hx[i >= 2 ? i - 2 : 0] : R(0);
+            ddy[i] = i >= 2 ? R(kHC.d2[i]) * o.hy[i >= 2 ? i - 2 : 0] : R(0);
         }
         R u0[S + 1], u1[S + 1], u2[S + 1];
 #pragma unroll
@@ -409,21 +423,31 @@ struct Walker {
         for (int k = 0; k < NP; k++) {
             const int nx = orb_nx(k), ny = orb_ny(k);
             const R B = col[k];
-            u0[nx] += B * hy[ny];
-            if (ny >= 1) u1[nx] += B * dy[ny];
+            u0[nx] += B * o.hy[ny];
+            if (ny >= 1) u1[nx] += B * o.dy[ny];
             if (ny >= 2) u2[nx] += B * ddy[ny];
         }
         R ax = R(0), ay = R(0), ss = R(0);
 #pragma unroll
         for (int i = 0; i <= S; i++) {
-            if (i >= 1) ax += dx[i] * u0[i];
+            if (i >= 1) ax += o.dx[i] * u0[i];
             if (i >= 2) ss += ddx[i] * u0[i];
-            ay += hx[i] * u1[i];
-            ss += hx[i] * u2[i];
+            ay += o.hx[i] * u1[i];
+            ss += o.hx[i] * u2[i];
         }
         gx = s * ax;
         gy = s * ay;
         sec = s * s * ss;
+    }
+
+    // the same from the particle's position (px, py)
+    __device__ __forceinline__ void slater_derivs(const WalkParams& P, R px, R py,
+                                                  const R (&col)[NP], R& gx, R& gy, R& sec) const
+    {
+        const R s = sw(P);
+        OrbRows<R, S> o;
+        orb_rows<R, S>(s * px, s * py, o);
+        slater_derivs_rows(P, o, col, gx, gy, sec);
     }
 
     // FP64 Gauss-Jordan inverses of NBK spin blocks (b0, b0 + 1, ...) with
@@ -573,8 +597,8 @@ struct Walker {
                 const int q = s * NP + l;
                 if (!valid[s] || q == qq) continue;
                 const R e0 = x[s][0] - y0, e1 = x[s][1] - y1;
-                const R r = M<R>::fsqrt(e0 * e0 + e1 * e1);
-                const R ir = M<R>::rcp(r);
+                R r, ir;
+                M<R>::dist(e0 * e0 + e1 * e1, true, r, ir);
                 const R a = (upspin[s] == qq_up) ? a_same : a_anti;
                 const R f = M<R>::rcp(R(1) + gam * r);
                 const R u1 = a * f * f;
@@ -1020,6 +1044,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
 
     // Slater ratio R = phi(r_p') . Ainv[:,p] (ref slaterdeterminant.cpp:128-132)
     R Rs = R(1), Sl = R(0), gsn = R(0), iR = R(1);
+    OrbRows<R, WK::S> nrow;   // orbital rows of the proposed position
     if (P.has_slater) {
         // bring the moved spin block into a0 (wave-uniform branch: skipped when
         // no walker of the wave needs it)
@@ -1044,32 +1069,27 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
             w.act = slot;
         }
         const R s = w.sw(P);
-        R hx[WK::S + 1], hy[WK::S + 1];
-        hermite<R, WK::S>(s * xn0, hx);
-        hermite<R, WK::S>(s * xn1, hy);
+        orb_rows<R, WK::S>(s * xn0, s * xn1, nrow);
         // phi(r').col = sum_nx h_nx(x') u[nx], u[nx] = sum_ny h_ny(y') col(nx, ny); the
-        // derivative rows reuse u (d/dx') and v[ny] = sum_nx h_nx(x') col (d/dy')
+        // d/dx' row reuses u, and v[nx] = sum_ny h_ny'(y') col(nx, ny) gives the
+        // d/dy' row (both for exact Langevin only)
         const bool exact = P.sampling == kImportanceExact;
-        // v[nx] = sum_ny h_ny'(y') col(nx, ny) (the d/dy' row, exact Langevin only)
-        R dy[WK::S + 1];
-#pragma unroll
-        for (int i = 0; i <= WK::S; i++) dy[i] = i >= 1 ? R(kHC.d1[i]) * hy[i >= 1 ? i - 1 : 0] : R(0);
         R u[WK::S + 1], v[WK::S + 1];
 #pragma unroll
         for (int i = 0; i <= WK::S; i++) u[i] = v[i] = R(0);
 #pragma unroll
         for (int kk = 0; kk < NP; kk++) {
             const int nx = orb_nx(kk), ny = orb_ny(kk);
-            u[nx] += hy[ny] * w.a0[kk];
-            if (exact && ny >= 1) v[nx] += dy[ny] * w.a0[kk];
+            u[nx] += nrow.hy[ny] * w.a0[kk];
+            if (exact && ny >= 1) v[nx] += nrow.dy[ny] * w.a0[kk];
         }
         R Tx = R(0), Ty = R(0);
 #pragma unroll
         for (int i = 0; i <= WK::S; i++) {
-            Sl += hx[i] * u[i];
+            Sl += nrow.hx[i] * u[i];
             if (exact) {
-                if (i >= 1) Tx += (R(kHC.d1[i]) * hx[i >= 1 ? i - 1 : 0]) * u[i];
-                Ty += hx[i] * v[i];
+                if (i >= 1) Tx += nrow.dx[i] * u[i];
+                Ty += nrow.hx[i] * v[i];
             }
         }
         Rs = __shfl(Sl, osrc, 64);
