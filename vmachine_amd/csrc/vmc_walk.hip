@@ -384,7 +384,9 @@ struct Walker {
     // coordinate vector of the walker and its n = sigmoid(v) live in LDS.
     R rv[HJ], rn[HJ], rp[HJ];   // pre-activation v, n, p = 1 - n of the current state
     R pv[HJ], pn[HJ], pp[HJ];   // the same for the pending proposal
-    R rb[HJ], rc[HJ];           // b_h and sum_k W[k][h]^2
+    R* lbc;                     // b_h and sum_k W[k][h]^2 of the lane's units in LDS, [2][K][4]
+    __device__ __forceinline__ R rbv(int j) const { return lbc[4 * l + j]; }
+    __device__ __forceinline__ R rcv(int j) const { return lbc[4 * K + 4 * l + j]; }
     R ra[2][2];                 // RBM-Gaussian a of the owned coordinates
     R gR[2][2];                 // RBM-product d ln psi / dx of the owned coordinates
     int rk[2][2];               // rank of each owned coordinate in the sorted vector
@@ -791,8 +793,10 @@ struct Walker {
                 double c = 0;
                 if (h < H)
                     for (int k = 0; k < ND; k++) c += P.rbm_W[k * H + h] * P.rbm_W[k * H + h];
-                rb[j] = h < H ? R(P.rbm_b[h]) : R(0);
-                rc[j] = R(c);
+                if (g == 0) {   // the same for every walker of the wave
+                    lbc[4 * l + j] = h < H ? R(P.rbm_b[h]) : R(0);
+                    lbc[4 * K + 4 * l + j] = R(c);
+                }
             }
         }
 #pragma unroll
@@ -817,7 +821,7 @@ struct Walker {
 #pragma unroll
         for (int j = 0; j < HJ; j++) {
             const bool ok = hvalid(P, j);
-            rv[j] = rb[j] + isp2(P) * acc[j];
+            rv[j] = rbv(j) + isp2(P) * acc[j];
             const R e = M<R>::exp(rv[j]);
             rp[j] = ok ? M<R>::rcp(R(1) + e) : R(0);
             rn[j] = ok ? e * rp[j] : R(0);
@@ -852,7 +856,7 @@ struct Walker {
         }
         double t = 0;
 #pragma unroll
-        for (int j = 0; j < HJ; j++) t += double(rc[j]) * double(rn[j]) * double(rp[j]);
+        for (int j = 0; j < HJ; j++) t += double(rcv(j)) * double(rn[j]) * double(rp[j]);
         lr = P.isig2_p * P.isig2_p * gsum<K>(t);
     }
 
@@ -1159,7 +1163,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
 #pragma unroll
             for (int j = 0; j < WK::HJ; j++) {
                 const bool ok = w.hvalid(P, j);
-                w.pv[j] = w.rb[j] + w.isp2(P) * acc[j];
+                w.pv[j] = w.rbv(j) + w.isp2(P) * acc[j];
                 const R e = M<R>::exp(w.pv[j]);
                 w.pp[j] = ok ? M<R>::rcp(R(1) + e) : R(0);
                 w.pn[j] = ok ? e * w.pp[j] : R(0);
@@ -1851,6 +1855,7 @@ struct RbmLds {
     R* X;            // [G][32] matrix-core results, one row per walker
     R* xs;           // [G][2][ND] sorted coordinates: current, proposal
     R* n;            // [G][H]
+    R* bc;           // [2][K][4]: b_h and sum_k W[k][h]^2, unit h = l + K j at [l][j]
     // W_elsewhere: W in other LDS (the rebuild's), not in the dynamic area
     __device__ __forceinline__ void carve(double* base, int ND, int H, R* W_elsewhere = nullptr)
     {
@@ -1868,6 +1873,7 @@ struct RbmLds {
         X = p;
         xs = X + G * 32;
         n = xs + G * 2 * ND;
+        bc = n + G * H;
     }
 };
 
@@ -1876,7 +1882,7 @@ inline size_t rbm_lds_bytes(int K, int ND, int H, size_t rsize, bool w_elsewhere
     const int G = 64 / K;
     const size_t wsz = w_elsewhere ? 0 : (size_t)(ND > 32 ? ND : 32) * 4 * K;
     return sizeof(double) * (2 * ND * H + 2 * (ND + H) + G) +
-           rsize * (wsz + (size_t)G * 32 + (size_t)G * 2 * ND + (size_t)G * H);
+           rsize * (wsz + (size_t)G * 32 + (size_t)G * 2 * ND + (size_t)G * H + (size_t)8 * K);
 }
 
 // LDS the walk of one block needs besides the dynamic RBM / learned-proposal
@@ -1903,6 +1909,7 @@ __device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& 
     w.xs_cur = L.xs + (size_t)w.g * 2 * P.ND;
     w.xs_prop = w.xs_cur + P.ND;
     w.ln = L.n + (size_t)w.g * P.H;
+    w.lbc = L.bc;
 }
 
 // the gradient-window sums of dlnpsi/dW = xs n^T over the wave's walkers on
@@ -1911,7 +1918,10 @@ __device__ __forceinline__ void rbm_bind(Walker<R, NP, K, RB>& w, RbmLds<R, K>& 
 // instruction); FP32 products and 4-walker sums, each step's tile added to
 // the FP64 LDS sums (every (k, h) belongs to one lane).  Reads the staged
 // current sorted vectors, n and E_L of the live walkers.
-template <int K>
+// BT: row ND of the tiles (zero padding otherwise, ND % 16 != 0) is a row of
+// ones, so the same tiles also give the window sums of dlnpsi/db_h = n_h and of
+// E_L n_h over the wave's walkers, added to the b part of L.accAB
+template <int K, bool BT = false>
 __device__ __forceinline__ void rbm_mf_wgrad(const WalkParams& P, const RbmLds<float, K>& L, int glive)
 {
     constexpr int G = 64 / K, OPS = G / 4;
@@ -1933,7 +1943,7 @@ __device__ __forceinline__ void rbm_mf_wgrad(const WalkParams& P, const RbmLds<f
                 const bool live = gw < glive;
                 const float x = L.xs[gw * 2 * ND + (kr < ND ? kr : ND - 1)];
                 const float n = L.n[gw * H + (hc < H ? hc : H - 1)];
-                const float xa = (live && kr < ND) ? x : 0.f;
+                const float xa = (live && kr < ND) ? x : ((BT && live && kr == ND) ? 1.f : 0.f);
                 const float nb = (live && hc < H) ? n : 0.f;
                 d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, nb, d1, 0, 0, 0);
                 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(float(L.E[gw]) * xa, nb, d2, 0, 0, 0);
@@ -1944,6 +1954,10 @@ __device__ __forceinline__ void rbm_mf_wgrad(const WalkParams& P, const RbmLds<f
                 if (k < ND && hc < H) {
                     L.accW[k * H + hc] += double(d1[i]);
                     L.accW[ND * H + k * H + hc] += double(d2[i]);
+                }
+                if (BT && k == ND && hc < H) {
+                    L.accAB[ND + hc] += double(d1[i]);
+                    L.accAB[2 * ND + H + hc] += double(d2[i]);
                 }
             }
         }
@@ -2010,7 +2024,10 @@ __device__ __forceinline__ void density_sample(const Walker<R, NP, K, RB>& w, co
 // lds_row as walk_static_lds_doubles, rbm_dyn the dynamic area.  DN: the
 // instantiation counts electron densities when P.dens_flags asks for them
 // (kept out of the plain instantiations the walk runs without densities)
-template <typename R, int NP, int K, bool RB, bool LP, bool DN = false>
+// BT (FP32 matrix-core RBM walks with ND % 16 != 0): the window sums of
+// dlnpsi/db come from the W-gradient tiles (rbm_mf_wgrad) instead of per-lane
+// FP64 registers
+template <typename R, int NP, int K, bool RB, bool LP, bool DN = false, bool BT = false>
 __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D, int block,
                                            double* lds_T, double* lds_row, double* rbm_dyn)
 {
@@ -2152,10 +2169,12 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                     }
                     if (P.has_rbmp) {
                         // dlnpsi/db_h = n_h (ref rbmproduct.cpp:77-83)
+                        if constexpr (!BT) {
 #pragma unroll
-                        for (int j = 0; j < WK::HJ; j++) {
-                            rGb[j] += double(w.rn[j]);
-                            rGbE[j] += E * double(w.rn[j]);
+                            for (int j = 0; j < WK::HJ; j++) {
+                                rGb[j] += double(w.rn[j]);
+                                rGbE[j] += E * double(w.rn[j]);
+                            }
                         }
                         // dlnpsi/dW[k][h] = xs_k n_h, summed over the wave's walkers
                         if (w.l == 0) L.E[w.g] = E;
@@ -2163,7 +2182,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                         bool wg_done = false;
                         if constexpr (sizeof(R) == 4 && K >= 4) {
                             if (w.mf_on(P)) {
-                                rbm_mf_wgrad<K>(P, L, glive);
+                                rbm_mf_wgrad<K, BT>(P, L, glive);
                                 wg_done = true;
                             }
                         }
@@ -2235,7 +2254,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                                 L.accAB[nab + kc] += rGaE[s][d];
                             }
                 }
-                if (P.has_rbmp) {
+                if (P.has_rbmp && !BT) {
 #pragma unroll
                     for (int j = 0; j < WK::HJ; j++)
                         if (w.hvalid(P, j)) {
@@ -2325,7 +2344,10 @@ void walk_kernel(const WalkParams P)
         D.window_hi = S * (kb + 1);
         D.init_qf = st.fresh;
     }
-    walk_block<R, NP, K, RB, LP, DN>(P, D, blockIdx.x, lds, lds + (64 / K) * NP * NP, rbm_dyn);
+    // the RBM x Slater x Pade-Jastrow instantiations on tiles take the b sums
+    // from the tiles (their launcher checks ND % 16 != 0)
+    constexpr bool BT = RB && MF != 0 && SG >= 0 && sizeof(R) == 4;
+    walk_block<R, NP, K, RB, LP, DN, BT>(P, D, blockIdx.x, lds, lds + (64 / K) * NP * NP, rbm_dyn);
 }
 
 // ------------------------------------------------------------------------
@@ -2475,7 +2497,8 @@ static hipError_t launch_walk_t(const WalkParams& P, hipStream_t stream)
         // sampling with the tiles has instantiations of its own
         if constexpr (!DN) {
             const bool set = P.sampling == kImportanceCompat && P.has_slater == 1 && P.has_rbmg == 1 &&
-                             P.has_rbmp == 1 && P.has_pade == 1 && P.coulomb == 1 && P.has_gauss == 0;
+                             P.has_rbmp == 1 && P.has_pade == 1 && P.coulomb == 1 && P.has_gauss == 0 &&
+                             P.ND % 16 != 0;
             if (set && P.rbm_mf == 1) {
                 hipLaunchKernelGGL((walk_kernel<RT, NP, K, RB, LP, DN, 1, kImportanceCompat>),
                                    dim3(blocks), dim3(64), lds, stream, P);
