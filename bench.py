@@ -96,10 +96,10 @@ def timed(eng, iters, warmup, spi):
     return wall, kern / 1000.0, E / n, acc / n
 
 
-def measure_iat(eng, steps, block, chunk=1 << 14):
+def measure_iat(eng, steps, block, chunk=1 << 17):
     """tau_int of E_L of the engine's walkers (all of them recorded) over
-    `steps` further steps, recorded in chunks and reduced to means of `block`
-    steps on the host.  tau_int_sokal: Sokal's windowed sum over the
+    `steps` further steps, run in chunks whose sampled energies the engine
+    reduces to means of `block` steps on the device (vmc_run_blocked).  tau_int_sokal: Sokal's windowed sum over the
     autocorrelation about the mean pooled over all walkers (stats.iat_pooled),
     which needs series only a few windows long; tau_int_blocking: per-walker
     automated blocking, which reads low until each series is some hundred tau
@@ -107,13 +107,13 @@ def measure_iat(eng, steps, block, chunk=1 << 14):
     from vmachine_amd import stats
     W = eng.cfg.num_walkers
     Ys, s1, s2, cnt = [], 0.0, 0.0, 0
+    chunk = min(chunk, steps)
     for _ in range(max(1, steps // chunk)):
-        _, _, _, en = eng.run(chunk, 0, record_walkers=W)
-        e = en.astype(np.float64)
-        s1 += e.sum()
-        s2 += (e * e).sum()
-        cnt += e.size
-        Ys.append(stats.block_means(e, block))
+        sm, _, _, y = eng.run_blocked(chunk, block, record_walkers=W)
+        s1 += sm["energy"]
+        s2 += sm["energy2"]
+        cnt += sm["samples"]
+        Ys.append(y)
     Y = np.concatenate(Ys, axis=1)
     mean = s1 / cnt
     var_step = s2 / cnt - mean * mean
@@ -510,11 +510,11 @@ def main():
     ap.add_argument("--equil-steps", type=int, default=60000)
     ap.add_argument("--equil-dt", type=float, default=0.1)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--iat-steps", type=int, default=1 << 20,
+    ap.add_argument("--iat-steps", type=int, default=1 << 23,
                     help="recorded steps per walker for tau_int of the Langevin forms")
     ap.add_argument("--iat-walkers", type=int, default=2048,
                     help="walkers of the tau_int runs (every one recorded)")
-    ap.add_argument("--iat-block", type=int, default=64,
+    ap.add_argument("--iat-block", type=int, default=512,
                     help="steps per block mean of the recorded Langevin series")
     ap.add_argument("--lp-iat-steps", type=int, default=1 << 16)
     ap.add_argument("--lp-iat-block", type=int, default=4)

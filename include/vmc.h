@@ -154,6 +154,18 @@ int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration,
                      vmc_sums* sums, double* grad, double* grad_e, int record_walkers,
                      float* energies);
 
+/* As vmc_run_recorded, returning instead the means of `block` consecutive
+ * sampled local energies of each of the first `record_walkers` walkers,
+ * row-major [record_walkers][steps / block] (FP64; block >= 2 divides steps).
+ * The reduction runs on the device, so long series need no per-step copy to
+ * the host.  The block means are the input of the reference's blocking
+ * resampler, which reads every instant energy from the file
+ * Sampler::printInstantValuesToFile writes (ref src/sampler.cpp:624-632,
+ * include/block/blocker.cpp:15-33). */
+int vmc_run_blocked(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+                    vmc_sums* sums, double* grad, double* grad_e, int record_walkers, int block,
+                    double* means);
+
 /* As vmc_run_recorded with all four instant values of the first
  * `record_walkers` walkers: out [4][record_walkers][steps] (FP32) = local
  * energy, kinetic, external, interaction, the four series

@@ -99,3 +99,27 @@ def test_instant_values_are_the_sums():
     for q, key in enumerate(["energy", "kinetic", "external", "interaction"]):
         assert out[q].astype(np.float64).sum() == pytest.approx(sums[key], rel=1e-6)
     np.testing.assert_allclose(out[0], out[1] + out[2] + out[3], rtol=1e-5, atol=1e-5)
+
+
+def test_blocked_means_are_the_recorded_series():
+    """vmc_run_blocked: the device block means equal the means of the per-step
+    series vmc_run_recorded returns for the same chain (same positions, random
+    state and walk), and the run's sums are the recorded series' sums."""
+    with _engine(6, 40, 64) as e:
+        e.init_walkers(9)
+        e.run(50)
+        pos = e.positions()
+        seed, ctr = e.random_state()
+        sums_r, _, _, en = e.run(96, 4, record_walkers=40)
+        e.set_positions(pos)
+        e.set_random_state(seed, ctr)
+        sums_b, _, _, means = e.run_blocked(96, 8, 4, record_walkers=33)
+    assert means.shape == (33, 12)
+    ref = en[:33].astype(np.float64).reshape(33, 12, 8).mean(axis=2)
+    np.testing.assert_allclose(means, ref, rtol=1e-12, atol=0)
+    assert sums_b["energy"] == sums_r["energy"]
+    import vmachine_amd as V
+    with _engine(6, 8, 64) as e:
+        e.init_walkers(3)
+        with pytest.raises(V.VmcError, match="block"):
+            e.run_blocked(10, 3)

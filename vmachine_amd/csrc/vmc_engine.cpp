@@ -54,6 +54,8 @@ int batch_shape_id(int np, int K);
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
                                  hipStream_t stream);
+hipError_t launch_block_means(const float* rec, int walkers, int steps, int block, double* means,
+                              hipStream_t stream);
 hipError_t launch_fill(double* p, int n, double value, hipStream_t stream);
 hipError_t static_scratch(double** out);
 hipError_t launch_draws(int precision, uint64_t seed, uint32_t walker, uint64_t counter0, int nd,
@@ -169,6 +171,8 @@ struct vmc_engine {
     double* d_out = nullptr;
     float* d_rec = nullptr;
     size_t rec_cap = 0;
+    double* d_means = nullptr;   // block means of the recorded series
+    size_t means_cap = 0;
     // electron densities: one allocation, radial [bins] | spatial [bins^2] | pair [bins^2]
     unsigned long long* d_dens = nullptr;
     int dens_flags = 0, dens_bins = 0;
@@ -181,6 +185,7 @@ struct vmc_engine {
         if (d_acc) (void)hipFree(d_acc);
         if (d_out) (void)hipFree(d_out);
         if (d_rec) (void)hipFree(d_rec);
+        if (d_means) (void)hipFree(d_means);
         if (d_dens) (void)hipFree(d_dens);
         for (double* b : {d_rbm_a, d_rbm_b, d_rbm_W, d_rbm_acc, d_rbm_out})
             if (b) (void)hipFree(b);
@@ -511,7 +516,7 @@ static vmc::WalkParams base_params(const vmc_engine* e)
 
 static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batches,
                     vmc_sums* sums, double* grad, double* grad_e, int rec_walkers, float* energies,
-                    int rec_parts = 1)
+                    int rec_parts = 1, int rec_block = 1, double* means = nullptr)
 {
     if (!e) return fail(VMC_E_INVALID, "null engine");
     if (!e->params_set) return fail(VMC_E_STATE, "vmc_set_parameters has not been called");
@@ -533,6 +538,14 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
             e->rec_cap = 0;
             HIP_TRY(hipMalloc(&e->d_rec, need * sizeof(float)));
             e->rec_cap = need;
+        }
+        const size_t nm = rec_block > 1 ? (size_t)rec_walkers * (steps / rec_block) : 0;
+        if (nm > e->means_cap) {
+            if (e->d_means) (void)hipFree(e->d_means);
+            e->d_means = nullptr;
+            e->means_cap = 0;
+            HIP_TRY(hipMalloc(&e->d_means, nm * sizeof(double)));
+            e->means_cap = nm;
         }
     }
     vmc::WalkParams P = base_params(e);
@@ -570,7 +583,13 @@ static int run_impl(vmc_engine* e, int steps, int equil, int iteration, int batc
         HIP_TRY(hipMemcpyAsync(rbm_out.data(), e->d_rbm_out, sizeof(double) * e->rbm_width,
                                hipMemcpyDeviceToHost, e->stream));
     }
-    if (rec_walkers > 0 && energies)
+    if (rec_walkers > 0 && rec_block > 1) {
+        HIP_TRY(vmc::launch_block_means(e->d_rec, rec_walkers, steps, rec_block, e->d_means,
+                                        e->stream));
+        HIP_TRY(hipMemcpyAsync(means, e->d_means,
+                               sizeof(double) * (size_t)rec_walkers * (steps / rec_block),
+                               hipMemcpyDeviceToHost, e->stream));
+    } else if (rec_walkers > 0 && energies)
         HIP_TRY(hipMemcpyAsync(energies, e->d_rec,
                                sizeof(float) * (size_t)rec_parts * rec_walkers * steps,
                                hipMemcpyDeviceToHost, e->stream));
@@ -811,6 +830,19 @@ int vmc_run_recorded(vmc_engine* e, int steps, int equilibration, int iteration,
     if (record_walkers > 0 && !energies) return fail(VMC_E_INVALID, "null energies buffer");
     return run_impl(e, steps, equilibration, iteration, batches, sums, grad, grad_e,
                     record_walkers, energies);
+}
+
+int vmc_run_blocked(vmc_engine* e, int steps, int equilibration, int iteration, int batches,
+                    vmc_sums* sums, double* grad, double* grad_e, int record_walkers, int block,
+                    double* means)
+{
+    if (record_walkers < 1 || !means) return fail(VMC_E_INVALID, "record_walkers >= 1 and a means buffer");
+    if (block < 2 || steps % block != 0)
+        return fail(VMC_E_INVALID, "block >= 2 dividing steps");
+    if ((size_t)record_walkers * (size_t)(steps / block) >= (size_t)1 << 31)
+        return fail(VMC_E_INVALID, "record_walkers * steps / block must stay below 2^31");
+    return run_impl(e, steps, equilibration, iteration, batches, sums, grad, grad_e,
+                    record_walkers, nullptr, 1, block, means);
 }
 
 int vmc_run_instant(vmc_engine* e, int steps, int equilibration, int iteration, int batches,

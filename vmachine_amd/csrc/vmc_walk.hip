@@ -2797,6 +2797,31 @@ hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint6
                        nd, omega, seed);
     return hipGetLastError();
 }
+
+// means of `block` consecutive recorded values: one wave per (block, walker),
+// lane j sums values j, j + 64, ... in FP64, then a fixed butterfly over the
+// wave; rec [walkers][steps], means [walkers][steps / block]
+__global__ __launch_bounds__(64) void block_means_kernel(const float* rec, int steps, int block,
+                                                         double* means)
+{
+    const int nb = steps / block;
+    const int walker = blockIdx.x / nb, b = blockIdx.x - walker * nb;
+    const float* src = rec + (size_t)walker * steps + (size_t)b * block;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < block; j += 64) s += double(src[j]);
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m);
+    if (threadIdx.x == 0) means[(size_t)walker * nb + b] = s / double(block);
+}
+
+hipError_t launch_block_means(const float* rec, int walkers, int steps, int block, double* means,
+                              hipStream_t stream)
+{
+    // one workgroup per mean; the caller keeps walkers * (steps / block) < 2^31
+    hipLaunchKernelGGL(block_means_kernel, dim3(walkers * (steps / block)), dim3(64), 0, stream,
+                       rec, steps, block, means);
+    return hipGetLastError();
+}
 #endif
 
 }  // namespace vmc

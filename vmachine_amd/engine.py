@@ -129,6 +129,8 @@ def lib():
         L.vmc_lp_train.argtypes = [ctypes.c_void_p, _I, _I, _D, _DP]
         L.vmc_run_instant.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, ctypes.POINTER(VmcSums),
                                       _DP, _DP, _I, _FP]
+        L.vmc_run_blocked.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, ctypes.POINTER(VmcSums),
+                                      _DP, _DP, _I, _I, _DP]
         L.vmc_set_density.argtypes = [ctypes.c_void_p, _I, _I, _D]
         _I64P = ctypes.POINTER(ctypes.c_int64)
         L.vmc_get_density.argtypes = [ctypes.c_void_p, _I64P, _I64P, _I64P]
@@ -313,6 +315,23 @@ class Engine:
         n = self.E * self.maxpar
         return (sums.as_dict(), g[:n].reshape(self.E, self.maxpar),
                 ge[:n].reshape(self.E, self.maxpar), en)
+
+    def run_blocked(self, steps, block, equilibration=0, iteration=0, batches=1,
+                    record_walkers=None):
+        """vmc_run_blocked: (sums, g, ge, means [record_walkers][steps // block])
+        with means the FP64 averages of `block` consecutive sampled local
+        energies of each recorded walker (all walkers by default)."""
+        W = self.cfg.num_walkers if record_walkers is None else record_walkers
+        sums = VmcSums()
+        g = np.zeros(max(self.E * self.maxpar, 1))
+        ge = np.zeros_like(g)
+        means = np.zeros((W, steps // block), dtype=np.float64)
+        self._check(self.L.vmc_run_blocked(self.h, steps, equilibration, iteration, batches,
+                                           ctypes.byref(sums), _dp(g), _dp(ge), W, block,
+                                           _dp(means)))
+        n = self.E * self.maxpar
+        return (sums.as_dict(), g[:n].reshape(self.E, self.maxpar),
+                ge[:n].reshape(self.E, self.maxpar), means)
 
     def run_instant(self, steps, equilibration=0, iteration=0, batches=1, record_walkers=1):
         """vmc_run_instant: (sums, g, ge, out [4][record_walkers][steps]) with
