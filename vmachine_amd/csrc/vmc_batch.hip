@@ -54,30 +54,78 @@ __global__ __launch_bounds__(64) void batch_walk_kernel(const WalkParams P, int 
     walk_block<R, NP, K, false, false>(P, D, blockIdx.x, lds, lds + G * NP * NP, nullptr);
 }
 
-// per-system column sums of the per-walker accumulators, walkers in order;
-// out[iteration - it0][s][0..kNumWalkerAcc)
-__global__ __launch_bounds__(256) void batch_reduce_kernel(const BatchTable* table, const double* acc,
-                                                           double* out, const BatchState* st,
-                                                           int it0)
+// the number of 256-walker chunks of each system before system s (table
+// order) and of system s itself
+__device__ __forceinline__ int batch_chunk0(const BatchTable& T, int s)
+{
+    int c0 = 0;
+    for (int i = 0; i < s; i++) c0 += (T.sys[i].P.W + 255) / 256;
+    return c0;
+}
+
+// per-system column sums of the per-walker accumulators in the two stages of
+// the single-system reduction (reduce_acc_partial_kernel, reduce_acc_kernel),
+// so that a system's sums come out as a single-system engine's would: stage
+// one gives each 256-walker chunk of every system a workgroup (coalesced rows
+// into LDS, fixed tree), part[chunk][kNumWalkerAcc]
+__global__ __launch_bounds__(256) void batch_reduce_partial_kernel(const BatchTable* table,
+                                                                   const double* acc, double* part)
 {
     const BatchTable& T = *table;
-    __shared__ double part[256];
-    const int s = blockIdx.x;
+    __shared__ double tile[256 * kNumWalkerAcc];
+    int s = 0, c0 = 0;
+    for (; s < T.nsys; s++) {
+        const int nc = (T.sys[s].P.W + 255) / 256;
+        if ((int)blockIdx.x < c0 + nc) break;
+        c0 += nc;
+    }
+    if (s >= T.nsys) return;   // the launch covers exactly the chunks; uniform per workgroup
     const BatchSystem& B = T.sys[s];
-    const double* a0 = acc + (size_t)B.walker0 * kNumWalkerAcc;
-    double* o = out + ((size_t)(st->iteration - it0) * T.nsys + B.user) * kBatchOut;
-    for (int a = 0; a < kNumWalkerAcc; a++) {
-        double v = 0;
-        for (int i = threadIdx.x; i < B.P.W; i += 256) v += a0[(size_t)i * kNumWalkerAcc + a];
-        part[threadIdx.x] = v;
-        __syncthreads();
-        for (int m = 128; m > 0; m >>= 1) {
-            if ((int)threadIdx.x < m) part[threadIdx.x] += part[threadIdx.x + m];
-            __syncthreads();
+    const int w0 = ((int)blockIdx.x - c0) * 256;
+    const int n = (B.P.W - w0 < 256 ? B.P.W - w0 : 256) * kNumWalkerAcc;
+    const double* src = acc + ((size_t)B.walker0 + w0) * kNumWalkerAcc;
+    for (int i = threadIdx.x; i < 256 * kNumWalkerAcc; i += 256) tile[i] = i < n ? src[i] : 0.0;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m) {
+#pragma unroll
+            for (int a = 0; a < kNumWalkerAcc; a++)
+                tile[threadIdx.x * kNumWalkerAcc + a] += tile[(threadIdx.x + m) * kNumWalkerAcc + a];
         }
-        if (threadIdx.x == 0) o[a] = part[0];
         __syncthreads();
     }
+    if ((int)threadIdx.x < kNumWalkerAcc)
+        part[(size_t)blockIdx.x * kNumWalkerAcc + threadIdx.x] = tile[threadIdx.x];
+}
+
+// stage two: workgroup s sums system s's chunk partials in order with the same
+// tree; out[iteration - it0][user][0..kNumWalkerAcc)
+__global__ __launch_bounds__(256) void batch_reduce_kernel(const BatchTable* table,
+                                                           const double* part, double* out,
+                                                           const BatchState* st, int it0)
+{
+    const BatchTable& T = *table;
+    __shared__ double tile[256 * kNumWalkerAcc];
+    const int s = blockIdx.x;
+    const BatchSystem& B = T.sys[s];
+    const int c0 = batch_chunk0(T, s), nb = (B.P.W + 255) / 256;
+    const double* p0 = part + (size_t)c0 * kNumWalkerAcc;
+    for (int a = 0; a < kNumWalkerAcc; a++) {
+        double v = 0;
+        for (int i = threadIdx.x; i < nb; i += 256) v += p0[(size_t)i * kNumWalkerAcc + a];
+        tile[threadIdx.x * kNumWalkerAcc + a] = v;
+    }
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m) {
+#pragma unroll
+            for (int a = 0; a < kNumWalkerAcc; a++)
+                tile[threadIdx.x * kNumWalkerAcc + a] += tile[(threadIdx.x + m) * kNumWalkerAcc + a];
+        }
+        __syncthreads();
+    }
+    double* o = out + ((size_t)(st->iteration - it0) * T.nsys + B.user) * kBatchOut;
+    if ((int)threadIdx.x < kNumWalkerAcc) o[threadIdx.x] = tile[threadIdx.x];
 }
 
 // the reference's optimiser step on (alpha, gamma) of every system, from the
@@ -156,11 +204,14 @@ hipError_t launch_batch_walk(int precision, int shape, const WalkParams& P, int 
     return hipGetLastError();
 }
 
-hipError_t launch_batch_tail(const BatchTable* T, int nsys, BatchOpt* opt, double* theta,
-                             const double* acc, double* out, BatchState* st, int it0,
-                             int steps_total, hipStream_t stream)
+// part: [chunks][kNumWalkerAcc], chunks = the systems' 256-walker chunks in all
+hipError_t launch_batch_tail(const BatchTable* T, int nsys, int chunks, BatchOpt* opt,
+                             double* theta, const double* acc, double* part, double* out,
+                             BatchState* st, int it0, int steps_total, hipStream_t stream)
 {
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3(nsys), dim3(256), 0, stream, T, acc, out, st, it0);
+    hipLaunchKernelGGL(batch_reduce_partial_kernel, dim3(chunks), dim3(256), 0, stream, T, acc,
+                       part);
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(nsys), dim3(256), 0, stream, T, part, out, st, it0);
     hipLaunchKernelGGL(batch_adam_kernel, dim3(1), dim3(64), 0, stream, T, opt, theta, out, st, it0);
     hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, stream, st, steps_total);
     return hipGetLastError();

@@ -46,9 +46,9 @@ int lp_delta_floats();
 hipError_t launch_batch_walk(int precision, int shape, const WalkParams& P, int user, int t_begin,
                              int t_end, int acc_add, const BatchState* st, const double* theta,
                              hipStream_t stream);
-hipError_t launch_batch_tail(const BatchTable* T, int nsys, BatchOpt* opt, double* theta,
-                             const double* acc, double* out, BatchState* st, int it0,
-                             int steps_total, hipStream_t stream);
+hipError_t launch_batch_tail(const BatchTable* T, int nsys, int chunks, BatchOpt* opt,
+                             double* theta, const double* acc, double* part, double* out,
+                             BatchState* st, int it0, int steps_total, hipStream_t stream);
 int batch_shape_id(int np, int K);
 // out: [1 + ceil(W/256)][kNumWalkerAcc] (the sums, then block partials)
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
@@ -1236,12 +1236,14 @@ struct vmc_batch {
     vmc::BatchState* d_st = nullptr;
     vmc::BatchOpt* d_opt = nullptr;        // caller order
     double *d_theta = nullptr, *d_pos = nullptr, *d_qf = nullptr, *d_acc = nullptr;
+    double* d_part = nullptr;              // [chunks][kNumWalkerAcc]: reduction partials
+    int chunks = 0;                        // 256-walker chunks of all systems
     std::vector<size_t> pos_off;           // caller index -> first double of its positions
 
     ~vmc_batch()
     {
         for (void* p : {(void*)d_table, (void*)d_st, (void*)d_opt, (void*)d_theta, (void*)d_pos,
-                        (void*)d_qf, (void*)d_acc})
+                        (void*)d_qf, (void*)d_acc, (void*)d_part})
             if (p) (void)hipFree(p);
         for (hipEvent_t e : {ev0, ev1, fork})
             if (e) (void)hipEventDestroy(e);
@@ -1395,6 +1397,7 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         acc += (size_t)y.num_walkers;
     }
     b->total_walkers = (int)acc;
+    for (const auto& y : b->sys) b->chunks += (y.num_walkers + 255) / 256;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return cleanup(fail(VMC_E_DEVICE, "no HIP device visible"));
@@ -1406,6 +1409,7 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         (err = hipMalloc(&b->d_pos, sizeof(double) * pos)) != hipSuccess ||
         (err = hipMalloc(&b->d_qf, sizeof(double) * pos)) != hipSuccess ||
         (err = hipMalloc(&b->d_acc, sizeof(double) * acc * vmc::kNumWalkerAcc)) != hipSuccess ||
+        (err = hipMalloc(&b->d_part, sizeof(double) * b->chunks * vmc::kNumWalkerAcc)) != hipSuccess ||
         (err = hipMalloc(&b->d_theta, sizeof(double) * 2 * c.num_systems)) != hipSuccess ||
         (err = hipMalloc(&b->d_table, 2 * sizeof(vmc::BatchTable))) != hipSuccess ||
         (err = hipMalloc(&b->d_st, sizeof(vmc::BatchState))) != hipSuccess ||
@@ -1543,8 +1547,9 @@ int vmc_batch_run(vmc_batch* b, int iterations, int steps, int equilibration, in
         return done(fail(VMC_E_DEVICE, std::string("batch run: ") + hipGetErrorString(err)));
     auto enqueue = [&]() -> int {
         if (int rc = batch_enqueue_walk(b, total)) return rc;
-        HIP_TRY(vmc::launch_batch_tail(b->d_table, nsys, b->d_opt, b->d_theta, b->d_acc, d_out,
-                                       b->d_st, it0, total, b->stream));
+        HIP_TRY(vmc::launch_batch_tail(b->d_table, nsys, b->chunks, b->d_opt, b->d_theta,
+                                       b->d_acc, b->d_part, d_out, b->d_st, it0, total,
+                                       b->stream));
         return VMC_OK;
     };
     if (use_graph) {
