@@ -333,11 +333,12 @@ def batch_measure(args, V, Ws):
     Coulomb, alpha = gamma = 1, exact Langevin dt --dt with drift limit, FP32
     walk with an FP64 rebuild every --refresh steps), walkers per system Ws.  One optimisation iteration = --batch-steps steps per walker of every
     system, per-system sums and the device ADAM step (learning rate
-    --batch-lr).  value = walker-steps/s of all systems together, one
-    kernel per system per refresh segment, the kernels side by side on streams
-    of their own, the iteration replayed as a hipGraph,
-    shapes from the autotuner; beside it the same iterations with the default
-    shapes, without the graph, and as seven single-system engines run one after
+    --batch-lr).  value = walker-steps/s of all systems together, the
+    iteration replayed as a hipGraph, shapes and launch form from the
+    autotuner (one kernel per system per refresh segment side by side on
+    streams of their own, or one persistent launch per iteration); beside it
+    the same iterations with the default shapes, in the other launch form,
+    without the graph, and as seven single-system engines run one after
     another (host optimiser)."""
     S, lr = args.batch_steps, args.batch_lr
     systems = [dict(num_particles=N, omega=w, num_walkers=W, step_length=args.dt, alpha=1.0,
@@ -364,8 +365,24 @@ def batch_measure(args, V, Ws):
     res["default_shapes_graph"] = {"walker_steps_per_s": total_ws / wall,
                                    "kernel_walker_steps_per_s": total_ws / kern}
     print(f"batch default shapes: {total_ws / wall:.4g} walker-steps/s", file=sys.stderr, flush=True)
+    b.set_launch(V.BATCH_PERSISTENT)
+    wall, kern, _ = time_batch(b, True)
+    res["default_shapes_persistent_graph"] = {"walker_steps_per_s": total_ws / wall,
+                                              "kernel_walker_steps_per_s": total_ws / kern}
+    print(f"batch default shapes, persistent: {total_ws / wall:.4g} walker-steps/s",
+          file=sys.stderr, flush=True)
     lanes, ns = b.autotune(args.batch_tune_steps)
-    print(f"autotune lanes={lanes.tolist()} ns={ns.tolist()}", file=sys.stderr, flush=True)
+    launch = {V.BATCH_SIDE_BY_SIDE: "side_by_side", V.BATCH_PERSISTENT: "persistent"}[b.launch()]
+    print(f"autotune lanes={lanes.tolist()} ns={ns.tolist()} launch={launch}", file=sys.stderr,
+          flush=True)
+    tuned = b.launch()
+    for other, key in ((V.BATCH_SIDE_BY_SIDE, "side_by_side_graph"),
+                       (V.BATCH_PERSISTENT, "persistent_graph")):
+        b.set_launch(other)
+        wall_o, kern_o, _ = time_batch(b, True)
+        res[key] = {"walker_steps_per_s": total_ws / wall_o,
+                    "kernel_walker_steps_per_s": total_ws / kern_o}
+    b.set_launch(tuned)
     wall, kern, out = time_batch(b, True)
     res["graph"] = {"walker_steps_per_s": total_ws / wall, "kernel_walker_steps_per_s": total_ws / kern,
                     "ms_per_step": 1000.0 * wall / args.steps}
@@ -422,7 +439,8 @@ def batch_measure(args, V, Ws):
                       "walkers_per_system": Ws, "total_walkers": sum(Ws),
                       "steps_per_iteration": S, "dt": args.dt,
                       "refresh_interval": args.refresh, "learning_rate": lr},
-           "autotune": {"lanes_per_walker": lanes.tolist(), "ns_per_walker_step": ns.tolist()},
+           "autotune": {"lanes_per_walker": lanes.tolist(), "ns_per_walker_step": ns.tolist(),
+                        "launch": launch},
            "forms": res,
            "speedup_vs_sequential_engines": value / res["sequential_engines"]["walker_steps_per_s"],
            "energies_last_iteration": energies, "alpha_gamma": params}
@@ -678,9 +696,10 @@ def main():
                  "speedup_vs_sequential_engines": bl["speedup_vs_sequential_engines"],
                  "energies_last_iteration": bl["energies_last_iteration"],
                  "note": "one optimisation iteration = 200 steps per walker of every system + "
-                         "device ADAM; graph = side-by-side kernels replayed as a hipGraph; "
-                         "sequential = seven single-system engines in turn with the host "
-                         "optimiser"}
+                         "device ADAM, replayed as a hipGraph; launch = one kernel per system "
+                         "per refresh segment side by side, or one persistent launch per "
+                         "iteration (autotuned, both timed); sequential = seven single-system "
+                         "engines in turn with the host optimiser"}
 
     out = {
         "metric": "walker-steps/s and IAT-corrected samples/s at N=56 w=0.1, vs plain FP64",

@@ -365,8 +365,22 @@ int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker);
 /* Times each system's candidate shapes on its own walkers (positions and
  * random state are restored) and keeps the fastest; lanes_out [num_systems]
  * receives the choices, ns_per_step_out [num_systems] (may be null) their time
- * per walker-step, which then orders the launches. */
+ * per walker-step, which then orders the launches.  Where the configuration
+ * allows the persistent launch, it then times the whole batch side by side and
+ * as one persistent launch and keeps the faster (vmc_batch_get_launch). */
 int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, double* ns_per_step_out);
+/* How vmc_batch_run launches an iteration's walk.  VMC_BATCH_SIDE_BY_SIDE:
+ * per refresh segment one kernel per system, on streams of their own.
+ * VMC_BATCH_PERSISTENT: one launch of one persistent kernel per iteration,
+ * whose waves (as many as fit on the GPU at once) take tiles of 64/K walkers
+ * of one system from a device counter, costliest system's tiles first, and
+ * walk each through every refresh segment (FP32 Slater x Gaussian x
+ * Pade-Jastrow with Coulomb under exact Langevin; VMC_E_INVALID otherwise).
+ * Both give the same chains. */
+#define VMC_BATCH_SIDE_BY_SIDE 0
+#define VMC_BATCH_PERSISTENT 1
+int vmc_batch_set_launch(vmc_batch* b, int launch);
+int vmc_batch_get_launch(vmc_batch* b, int* launch);
 /* `iterations` optimisation iterations of equilibration + steps steps each.
  * out [iterations][num_systems][VMC_BATCH_OUT]; kernel_ms (may be null) = the
  * device time of all iterations.  learning_rate 0 keeps the parameters. */

@@ -169,6 +169,64 @@ def test_graph_replay_equals_direct_launches():
     np.testing.assert_array_equal(res[0][2], res[1][2])
 
 
+# all seven closed shells, ragged walker counts that leave partial tiles
+SYSTEMS_SEVEN = [(2, 83, 0.1), (6, 70, 0.05), (12, 37, 0.02), (20, 45, 0.01), (30, 29, 0.01),
+                 (42, 19, 0.005), (56, 13, 0.005)]
+
+
+@pytest.mark.parametrize("refresh", [40, 100])
+def test_persistent_launch_equals_side_by_side(refresh):
+    """One persistent launch per iteration (waves taking tiles of one system
+    from a device counter, every refresh segment of a tile in one go) walks
+    the same chains as the side-by-side kernels: the same walk code and draws
+    per walker, compiled as a called function instead of a kernel body, so the
+    FP32 results agree to rounding (sums 1e-5 relative, parameters after ADAM
+    1e-6, >= 95 % of the walkers' positions within 1e-4, as in the FP32
+    comparisons with single-system engines), and the persistent launch as a
+    graph equals its direct launches bit for bit.  Configurations the
+    persistent kernel does not carry are refused."""
+    V = _require_gpu()
+    rng = np.random.default_rng(21)
+    systems = _systems(rng, SYSTEMS_SEVEN)
+    res = []
+    for launch, use_graph in ((V.BATCH_SIDE_BY_SIDE, False), (V.BATCH_PERSISTENT, False),
+                              (V.BATCH_PERSISTENT, True)):
+        print(f"persistent test: refresh={refresh} launch={launch} graph={use_graph}", flush=True)
+        with _batch(V, systems, O.IMPORTANCE_EXACT, precision=32, quirks=0, drift=1.0,
+                    refresh=refresh) as b:
+            b.set_launch(launch)
+            assert b.launch() == launch
+            b.init_walkers(606)
+            out, ms = b.run(3, 120, 10, batches=2, learning_rate=0.02, use_graph=use_graph)
+            assert ms > 0
+            res.append((out, [b.get_positions(s) for s in range(len(systems))], b.parameters()))
+    assert np.all(np.isfinite(res[0][0])) and np.all(res[0][0][:, :, 13] == 0)
+    side, pers, pers_graph = res
+    np.testing.assert_array_equal(pers_graph[0], pers[0])
+    for a, c in zip(pers_graph[1], pers[1]):
+        np.testing.assert_array_equal(a, c)
+    np.testing.assert_array_equal(pers_graph[2], pers[2])
+    # sums of E, T, V_ext, V_int cancel between walkers far from equilibrium
+    # (T < 0 near a node): their scale is sqrt(M * sum of squares), not the sum
+    squares = {0: 1, 2: 5, 3: 6, 4: 7}
+    for it in range(side[0].shape[0]):
+        for s in range(len(systems)):
+            o, r = pers[0][it, s], side[0][it, s]
+            M = systems[s]["num_walkers"] * 130
+            for k in range(13):
+                scale = np.sqrt(M * r[squares[k]]) if k in squares else abs(r[k])
+                assert abs(o[k] - r[k]) <= 1e-5 * max(1.0, scale), (it, s, k, o[k], r[k])
+            assert o[13] == r[13] == 0
+            np.testing.assert_allclose(o[14:16], r[14:16], rtol=1e-6)
+    for s in range(len(systems)):
+        close = np.all(np.abs(pers[1][s] - side[1][s]) < 1e-4, axis=1)
+        assert close.mean() >= 0.95, (s, close.mean())
+    with _batch(V, systems[:2], O.IMPORTANCE_EXACT, precision=64) as b:
+        with pytest.raises(V.VmcError, match="persistent"):
+            b.set_launch(V.BATCH_PERSISTENT)
+        assert b.launch() == V.BATCH_SIDE_BY_SIDE
+
+
 def test_device_adam_matches_host_optimiser():
     """Three iterations of lr 0.05: (alpha, gamma) after every device ADAM step
     against the host optimiser driven by single-engine sums."""
@@ -227,6 +285,7 @@ def test_shapes_and_autotune():
             K = LANES[ORBITALS[y["num_particles"]]]
             assert lanes[s] in (K, 2 * K) and ns[s] > 0
             np.testing.assert_array_equal(b.get_positions(s), before[s])
+        assert b.launch() in (V.BATCH_SIDE_BY_SIDE, V.BATCH_PERSISTENT)
         out, _ = b.run(2, 100, 10)
         assert np.all(np.isfinite(out))
         for s, y in enumerate(systems):

@@ -54,6 +54,118 @@ __global__ __launch_bounds__(64) void batch_walk_kernel(const WalkParams P, int 
     walk_block<R, NP, K, false, false>(P, D, blockIdx.x, lds, lds + G * NP * NP, nullptr);
 }
 
+// ---- one persistent launch per iteration ----
+// A tile is one wave's block of 64/K walkers of one system, walked through
+// every refresh segment of the iteration (a rebuild at the start of each).
+// Tiles are numbered in table order, costliest system first, and the waves of
+// one grid that fills the GPU take them from a device counter: the big
+// systems' tiles start first and the small systems' tiles fill the tail, with
+// no launch boundary between systems or refresh segments.  Each shape's walk
+// is a function of its own, called rather than inlined, so the kernel's
+// register count is the largest shape's, not that of all shapes' code merged
+// into one body.
+
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T*)(((uint64_t)hi << 32) | lo);
+}
+
+// largest static LDS of the batch shapes (N=56: 2 walkers x 28 x 28 + rows)
+constexpr int batch_lds_doubles()
+{
+    int m = 0;
+#define X(id, np, k) m = m > walk_static_lds_doubles<np, k>() ? m : walk_static_lds_doubles<np, k>();
+    VMC_BATCH_SHAPES(X)
+#undef X
+    return m;
+}
+
+// the walk of tile `tile` of system s (table order) through all segments; the
+// arguments are wave-uniform, which readfirstlane tells the compiler
+template <int NP, int K>
+__device__ __attribute__((noinline)) void batch_tile(const BatchTable* table, int s, int tile,
+                                                     const BatchState* state,
+                                                     const double* theta_in, int total, int seg,
+                                                     double* lds)
+{
+    // the table is read through the constant address space at a uniform
+    // address, so the walk's parameter loads are scalar loads, as from a kernel
+    // argument
+    typedef const __attribute__((address_space(4))) BatchTable ConstTable;
+    s = __builtin_amdgcn_readfirstlane(s);
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    constexpr int G = 64 / K;
+    for (int t0 = 0; t0 < total; t0 += seg) {
+        // the pointers are made opaque once per segment: nothing derived from
+        // them is hoisted out of the segment loop and held live across the
+        // walk (which spilled registers)
+        uint64_t tv = (uint64_t)uniform_ptr(table);
+        uint64_t sv = (uint64_t)uniform_ptr(state);
+        uint64_t hv = (uint64_t)uniform_ptr(theta_in);
+        asm volatile("" : "+s"(tv), "+s"(sv), "+s"(hv));
+        const ConstTable* T = (const ConstTable*)tv;
+        const BatchState* __restrict__ st = (const BatchState*)sv;
+        const double* __restrict__ theta = (const double*)hv;
+        const WalkParams& P = (const WalkParams&)T->sys[s].P;
+        __builtin_assume(P.sampling == kImportanceExact);
+        __builtin_assume(P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1 && P.coulomb == 1);
+        const int user = T->sys[s].user;
+        WalkDyn D;
+        D.alpha = theta[2 * user];
+        D.gamma = theta[2 * user + 1];
+        D.counter0 = st->counter0;
+        const int S = P.steps / st->batches;                 // ref sampler.cpp:104
+        const int kb = st->iteration % st->batches;
+        D.window_lo = S * kb;
+        D.window_hi = S * (kb + 1);
+        D.init_qf = st->fresh;
+        D.t_begin = t0;
+        D.t_end = t0 + seg < total ? t0 + seg : total;
+        D.acc_add = t0 > 0 ? 1 : 0;
+        walk_block<float, NP, K, false, false>(P, D, tile, lds, lds + G * NP * NP, nullptr);
+    }
+}
+
+// FP32, Slater x Gaussian x Pade-Jastrow, Coulomb, exact Langevin; every wave
+// takes tiles until the counter passes the last one, then leaves
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void batch_persistent_kernel(const BatchTable* __restrict__ T, int* queue,
+                             const BatchState* st, const double* theta, int total, int seg)
+{
+    __shared__ double lds[batch_lds_doubles()];
+    const int nsys = T->nsys;
+    for (;;) {
+        int t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(queue, 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        int s = 0, t0 = 0;
+        for (; s < nsys; s++) {
+            if (t < t0 + T->sys[s].blocks) break;
+            t0 += T->sys[s].blocks;
+        }
+        if (s >= nsys) break;
+        switch (T->sys[s].shape) {
+#define X(id, np, k)                                                                \
+    case id:                                                                        \
+        batch_tile<np, k>(T, s, t - t0, st, theta, total, seg, lds);                \
+        break;
+            VMC_BATCH_SHAPES(X)
+#undef X
+        default:
+            break;
+        }
+    }
+}
+
+__global__ void batch_queue_reset_kernel(int* queue)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *queue = 0;
+}
+
 // the number of 256-walker chunks of each system before system s (table
 // order) and of system s itself
 __device__ __forceinline__ int batch_chunk0(const BatchTable& T, int s)
@@ -202,6 +314,33 @@ hipError_t launch_batch_walk(int precision, int shape, const WalkParams& P, int 
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// one iteration's walk as one persistent launch (the counter reset first, by a
+// kernel, so that the pair can be captured in a graph); grid: resident waves
+hipError_t launch_batch_persistent(const BatchTable* T, int* queue, const BatchState* st,
+                                   const double* theta, int total, int seg, int grid,
+                                   hipStream_t stream)
+{
+    hipLaunchKernelGGL(batch_queue_reset_kernel, dim3(1), dim3(64), 0, stream, queue);
+    hipLaunchKernelGGL(batch_persistent_kernel, dim3(grid), dim3(64), 0, stream, T, queue, st,
+                       theta, total, seg);
+    return hipGetLastError();
+}
+
+// workgroups of the persistent kernel resident on the whole GPU at once
+hipError_t batch_persistent_grid(int* grid)
+{
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    hipError_t err;
+    if ((err = hipGetDevice(&dev)) != hipSuccess ||
+        (err = hipGetDeviceProperties(&prop, dev)) != hipSuccess ||
+        (err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_persistent_kernel, 64,
+                                                             0)) != hipSuccess)
+        return err;
+    *grid = (per_cu < 1 ? 1 : per_cu) * prop.multiProcessorCount;
+    return hipSuccess;
 }
 
 // part: [chunks][kNumWalkerAcc], chunks = the systems' 256-walker chunks in all

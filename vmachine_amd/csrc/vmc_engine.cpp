@@ -50,6 +50,10 @@ hipError_t launch_batch_tail(const BatchTable* T, int nsys, int chunks, BatchOpt
                              double* theta, const double* acc, double* part, double* out,
                              BatchState* st, int it0, int steps_total, hipStream_t stream);
 int batch_shape_id(int np, int K);
+hipError_t launch_batch_persistent(const BatchTable* T, int* queue, const BatchState* st,
+                                   const double* theta, int total, int seg, int grid,
+                                   hipStream_t stream);
+hipError_t batch_persistent_grid(int* grid);
 // out: [1 + ceil(W/256)][kNumWalkerAcc] (the sums, then block partials)
 hipError_t launch_reduce(const double* acc, int W, double* out, hipStream_t stream);
 hipError_t launch_init_positions(double* pos, int W, int nd, double omega, uint64_t seed,
@@ -1238,12 +1242,16 @@ struct vmc_batch {
     double *d_theta = nullptr, *d_pos = nullptr, *d_qf = nullptr, *d_acc = nullptr;
     double* d_part = nullptr;              // [chunks][kNumWalkerAcc]: reduction partials
     int chunks = 0;                        // 256-walker chunks of all systems
+    int launch = VMC_BATCH_SIDE_BY_SIDE;   // how an iteration's walk is launched
+    int* d_queue = nullptr;                // the persistent launch's tile counter
+    int persistent_grid = 0;               // its resident workgroups
     std::vector<size_t> pos_off;           // caller index -> first double of its positions
+    size_t pos_total = 0;                  // doubles of all systems' positions
 
     ~vmc_batch()
     {
         for (void* p : {(void*)d_table, (void*)d_st, (void*)d_opt, (void*)d_theta, (void*)d_pos,
-                        (void*)d_qf, (void*)d_acc, (void*)d_part})
+                        (void*)d_qf, (void*)d_acc, (void*)d_part, (void*)d_queue})
             if (p) (void)hipFree(p);
         for (hipEvent_t e : {ev0, ev1, fork})
             if (e) (void)hipEventDestroy(e);
@@ -1272,6 +1280,16 @@ void batch_plan(vmc_batch* b)
     }
 }
 
+// the persistent launch carries FP32 walks of Slater x Gaussian x Pade-Jastrow
+// with Coulomb under exact Langevin (the mixed-size batch's configuration)
+bool batch_persistent_ok(const vmc_batch* b)
+{
+    if (b->cfg.precision != 32 || b->cfg.sampling != VMC_IMPORTANCE_EXACT || !b->cfg.interaction)
+        return false;
+    const vmc::WalkParams& P = b->hs[0].P;
+    return P.has_slater == 1 && P.has_gauss == 1 && P.has_pade == 1;
+}
+
 vmc::BatchTable batch_table(const vmc_batch* b)
 {
     vmc::BatchTable T{};
@@ -1288,6 +1306,11 @@ vmc::BatchTable batch_table(const vmc_batch* b)
 int batch_enqueue_walk(vmc_batch* b, int total)
 {
     const int seg = b->cfg.refresh_interval > 0 ? b->cfg.refresh_interval : total;
+    if (b->launch == VMC_BATCH_PERSISTENT) {
+        HIP_TRY(vmc::launch_batch_persistent(b->d_table, b->d_queue, b->d_st, b->d_theta, total, seg,
+                                             b->persistent_grid, b->stream));
+        return VMC_OK;
+    }
     const int n = (int)b->hs.size();
     for (int t0 = 0; t0 < total; t0 += seg) {
         const int t1 = t0 + seg < total ? t0 + seg : total;
@@ -1397,6 +1420,7 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         acc += (size_t)y.num_walkers;
     }
     b->total_walkers = (int)acc;
+    b->pos_total = pos;
     for (const auto& y : b->sys) b->chunks += (y.num_walkers + 255) / 256;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -1410,6 +1434,7 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         (err = hipMalloc(&b->d_qf, sizeof(double) * pos)) != hipSuccess ||
         (err = hipMalloc(&b->d_acc, sizeof(double) * acc * vmc::kNumWalkerAcc)) != hipSuccess ||
         (err = hipMalloc(&b->d_part, sizeof(double) * b->chunks * vmc::kNumWalkerAcc)) != hipSuccess ||
+        (err = hipMalloc(&b->d_queue, sizeof(int))) != hipSuccess ||
         (err = hipMalloc(&b->d_theta, sizeof(double) * 2 * c.num_systems)) != hipSuccess ||
         (err = hipMalloc(&b->d_table, 2 * sizeof(vmc::BatchTable))) != hipSuccess ||
         (err = hipMalloc(&b->d_st, sizeof(vmc::BatchState))) != hipSuccess ||
@@ -1509,6 +1534,28 @@ int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker)
     B.K = lanes_per_walker;
     B.shape = id;
     batch_plan(b);
+    return VMC_OK;
+}
+
+int vmc_batch_set_launch(vmc_batch* b, int launch)
+{
+    if (!b) return fail(VMC_E_INVALID, "null batch");
+    if (launch != VMC_BATCH_SIDE_BY_SIDE && launch != VMC_BATCH_PERSISTENT)
+        return fail(VMC_E_INVALID, "launch: VMC_BATCH_SIDE_BY_SIDE or VMC_BATCH_PERSISTENT");
+    if (launch == VMC_BATCH_PERSISTENT) {
+        if (!batch_persistent_ok(b))
+            return fail(VMC_E_INVALID, "the persistent launch walks FP32 Slater x Gaussian x "
+                                       "Pade-Jastrow with Coulomb under exact Langevin only");
+        if (!b->persistent_grid) HIP_TRY(vmc::batch_persistent_grid(&b->persistent_grid));
+    }
+    b->launch = launch;
+    return VMC_OK;
+}
+
+int vmc_batch_get_launch(vmc_batch* b, int* launch)
+{
+    if (!b || !launch) return fail(VMC_E_INVALID, "null argument");
+    *launch = b->launch;
     return VMC_OK;
 }
 
@@ -1649,6 +1696,48 @@ int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, double* ns_per_s
         if (ns_per_step_out) ns_per_step_out[B.user] = best;
     }
     batch_plan(b);
+    if (!batch_persistent_ok(b)) return finish(VMC_OK);
+    // the launch: the whole batch walked `steps` steps side by side and as one
+    // persistent launch (second run of each timed), from the same positions
+    // and drifts; the faster is kept
+    (void)hipFree(d_save);
+    d_save = nullptr;
+    if ((!b->persistent_grid && (err = vmc::batch_persistent_grid(&b->persistent_grid)) != hipSuccess) ||
+        (err = hipMalloc(&d_save, sizeof(double) * 2 * b->pos_total)) != hipSuccess)
+        return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+    for (auto& B : b->hs) {
+        B.P.seed = b->seed + (uint64_t)B.user;
+        B.P.steps = steps;
+        B.P.equil = 0;
+    }
+    const vmc::BatchTable T = batch_table(b);
+    const size_t np = b->pos_total;
+    if ((err = hipMemcpy(b->d_table, &T, sizeof(T), hipMemcpyHostToDevice)) != hipSuccess ||
+        (err = hipMemcpy(d_save, b->d_pos, sizeof(double) * np, hipMemcpyDeviceToDevice)) != hipSuccess ||
+        (err = hipMemcpy(d_save + np, b->d_qf, sizeof(double) * np, hipMemcpyDeviceToDevice)) != hipSuccess)
+        return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+    double best_ms = 1e300;
+    int best_launch = VMC_BATCH_SIDE_BY_SIDE;
+    for (int launch : {VMC_BATCH_SIDE_BY_SIDE, VMC_BATCH_PERSISTENT}) {
+        b->launch = launch;
+        float ms = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            if ((err = hipEventRecord(b->ev0, b->stream)) != hipSuccess)
+                return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+            if (int rc = batch_enqueue_walk(b, steps)) return finish(rc);
+            if ((err = hipEventRecord(b->ev1, b->stream)) != hipSuccess ||
+                (err = hipStreamSynchronize(b->stream)) != hipSuccess ||
+                (err = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess ||
+                (err = hipMemcpy(b->d_pos, d_save, sizeof(double) * np, hipMemcpyDeviceToDevice)) != hipSuccess ||
+                (err = hipMemcpy(b->d_qf, d_save + np, sizeof(double) * np, hipMemcpyDeviceToDevice)) != hipSuccess)
+                return finish(fail(VMC_E_DEVICE, std::string("autotune: ") + hipGetErrorString(err)));
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_launch = launch;
+        }
+    }
+    b->launch = best_launch;
     return finish(VMC_OK);
 }
 

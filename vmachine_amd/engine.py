@@ -78,6 +78,8 @@ class VmcBatchConfig(ctypes.Structure):
 
 BATCH_OUT = 16   # per system and iteration: the 14 walk sums (VMC_EVAL order), alpha, gamma
 BATCH_MAX_SYSTEMS = 8
+BATCH_SIDE_BY_SIDE = 0   # vmc_batch_set_launch: one kernel per system per refresh segment
+BATCH_PERSISTENT = 1     # one persistent launch per iteration
 
 
 class LpHyper(ctypes.Structure):
@@ -148,6 +150,8 @@ def lib():
         L.vmc_batch_get_parameters.argtypes = [ctypes.c_void_p, _DP]
         L.vmc_batch_set_shape.argtypes = [ctypes.c_void_p, _I, _I]
         L.vmc_batch_autotune.argtypes = [ctypes.c_void_p, _I, _IP, _DP]
+        L.vmc_batch_set_launch.argtypes = [ctypes.c_void_p, _I]
+        L.vmc_batch_get_launch.argtypes = [ctypes.c_void_p, _IP]
         L.vmc_batch_run.argtypes = [ctypes.c_void_p, _I, _I, _I, _I, _D, _I, _DP, _DP]
         _lib = L
     return _lib
@@ -503,6 +507,16 @@ class Batch:
 
     def set_shape(self, s, lanes_per_walker):
         self._check(self.L.vmc_batch_set_shape(self.h, s, lanes_per_walker))
+
+    def set_launch(self, launch):
+        """BATCH_SIDE_BY_SIDE (one kernel per system per refresh segment) or
+        BATCH_PERSISTENT (one persistent launch per iteration)."""
+        self._check(self.L.vmc_batch_set_launch(self.h, int(launch)))
+
+    def launch(self):
+        v = ctypes.c_int(0)
+        self._check(self.L.vmc_batch_get_launch(self.h, ctypes.byref(v)))
+        return v.value
 
     def autotune(self, steps):
         """(lanes per walker, ns per walker-step) chosen for every system."""
