@@ -367,7 +367,9 @@ int vmc_batch_set_shape(vmc_batch* b, int system, int lanes_per_walker);
  * receives the choices, ns_per_step_out [num_systems] (may be null) their time
  * per walker-step, which then orders the launches.  Where the configuration
  * allows the persistent launch, it then times the whole batch side by side and
- * as one persistent launch and keeps the faster (vmc_batch_get_launch). */
+ * as one persistent launch over steps + min(refresh_interval, 4 steps) steps
+ * (a trial that crosses a refresh boundary) and keeps the faster
+ * (vmc_batch_get_launch). */
 int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, double* ns_per_step_out);
 /* How vmc_batch_run launches an iteration's walk.  VMC_BATCH_SIDE_BY_SIDE:
  * per refresh segment one kernel per system, on streams of their own.
