@@ -308,6 +308,16 @@ __device__ __forceinline__ void sfor(F&& f)
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 mf_b8 __attribute__((ext_vector_type(8)));
 
+// LDS row strides of the RBM state, padded so that the lanes of a tile operand
+// (one row each) fall into different banks: the walk's copy of W, [32][4K + 1]
+// (unit h = l + K j of row k at k (4K + 1) + 4 l + j); the tile results, one row
+// of 36 per walker (16-byte aligned rows for the four-element stores); the
+// FP64 window sums of xs_k n_h, row k of H + 1
+template <int K>
+__host__ __device__ constexpr int rbm_wst() { return 4 * K + 1; }
+constexpr int kMfX = 36;
+__host__ __device__ constexpr int rbm_hst(int H) { return H + 1; }
+
 template <int MODE>
 __device__ __forceinline__ int mf_index(int q, int e) { return MODE == 1 ? 4 * e + q : 8 * q + e; }
 
@@ -396,7 +406,7 @@ struct Walker {
     R* xs_cur;                  // this walker's sorted coordinates
     R* xs_prop;                 // the proposal's sorted coordinates (scratch)
     R* ln;                      // this walker's n_h, h = 0..H-1
-    R* mf_x;                    // [G][32] matrix-core results of the wave's walkers
+    R* mf_x;                    // [G][kMfX] matrix-core results of the wave's walkers
     R* mf_xs;                   // every walker's sorted vectors, [G][2][ND]
     R* mf_n;                    // every walker's n, [G][H]
 
@@ -717,15 +727,15 @@ struct Walker {
                     const int ho = 4 * (h % K) + h / K;
                     float a[8];
 #pragma unroll
-                    for (int e = 0; e < 8; e++) a[e] = lW[mf_index<MODE>(q, e) * 4 * K + ho];
+                    for (int e = 0; e < 8; e++) a[e] = lW[mf_index<MODE>(q, e) * rbm_wst<K>() + ho];
                     const mf_f4 d = mf_tile<MODE>(a, b);
-                    if (c < G) *reinterpret_cast<mf_f4*>(mf_x + c * 32 + 16 * th + 4 * q) = d;
+                    if (c < G) *reinterpret_cast<mf_f4*>(mf_x + c * kMfX + 16 * th + 4 * q) = d;
                 }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < HJ; j++) {
                 const int h = l + K * j;
-                acc[j] = hvalid(P, j) ? mf_x[g * 32 + (h < 32 ? h : 31)] : R(0);
+                acc[j] = hvalid(P, j) ? mf_x[g * kMfX + (h < 32 ? h : 31)] : R(0);
             }
             __syncthreads();
         }
@@ -756,11 +766,11 @@ struct Walker {
                     for (int e = 0; e < 8; e++) {
                         const int h = mf_index<MODE>(q, e);
                         const int hc = h < 4 * K ? h : 4 * K - 1;
-                        const float v = lW[k * 4 * K + 4 * (hc % K) + hc / K];
+                        const float v = lW[k * rbm_wst<K>() + 4 * (hc % K) + hc / K];
                         a[e] = h < 4 * K ? v : 0.f;
                     }
                     const mf_f4 d = mf_tile<MODE>(a, b);
-                    if (c < G) *reinterpret_cast<mf_f4*>(mf_x + c * 32 + 16 * tk + 4 * q) = d;
+                    if (c < G) *reinterpret_cast<mf_f4*>(mf_x + c * kMfX + 16 * tk + 4 * q) = d;
                 }
             __syncthreads();
 #pragma unroll
@@ -769,7 +779,7 @@ struct Walker {
                 for (int d = 0; d < 2; d++)
                     {
                     const int k = 2 * (s * NP + l) + d;
-                    gR[s][d] = valid[s] ? isp2(P) * mf_x[g * 32 + (k < 32 ? k : 31)] : R(0);
+                    gR[s][d] = valid[s] ? isp2(P) * mf_x[g * kMfX + (k < 32 ? k : 31)] : R(0);
                 }
             __syncthreads();
         }
@@ -779,13 +789,13 @@ struct Walker {
     // (block-wide, before any other RBM call)
     __device__ __forceinline__ void rbm_setup(const WalkParams& P)
     {
-        const int ND = P.ND, H = P.H, K4 = 4 * K;
+        const int ND = P.ND, H = P.H, K4 = 4 * K, WS = rbm_wst<K>();
         if (P.has_rbmp) {
             const int rows = ND > 32 ? ND : 32;   // zero rows pad the matrix-core tiles
-            for (int i = threadIdx.x; i < rows * K4; i += 64) {
-                const int k = i / K4, r = i - k * K4;
+            for (int i = threadIdx.x; i < rows * WS; i += 64) {
+                const int k = i / WS, r = i - k * WS;
                 const int h = (r >> 2) + K * (r & 3);
-                lW[i] = (h < H && k < ND) ? R(P.rbm_W[k * H + h]) : R(0);
+                lW[i] = (r < K4 && h < H && k < ND) ? R(P.rbm_W[k * H + h]) : R(0);
             }
 #pragma unroll
             for (int j = 0; j < HJ; j++) {
@@ -809,7 +819,7 @@ struct Walker {
     // v, n, p of the lane's units from the current sorted vector
     __device__ __forceinline__ void rbm_activate(const WalkParams& P)
     {
-        const int K4 = 4 * K;
+        const int K4 = rbm_wst<K>();
         R acc[HJ];
 #pragma unroll
         for (int j = 0; j < HJ; j++) acc[j] = R(0);
@@ -834,7 +844,7 @@ struct Walker {
     // Reads the walker's n from LDS: call after a barrier that follows the staging.
     __device__ __forceinline__ void rbm_refresh_grad(const WalkParams& P)
     {
-        const int K4 = 4 * K;
+        const int K4 = rbm_wst<K>();
         if (mf_on(P)) {
             if (P.rbm_mf == 2)
                 rbm_mf_grad<2>(P);
@@ -1132,7 +1142,7 @@ __device__ __forceinline__ bool walker_step(Walker<R, NP, K, RB>& w, const WalkP
             w.mk = k;
             w.r_old = ro;
             w.r_new = rq;
-            constexpr int K4 = 4 * K;
+            constexpr int K4 = rbm_wst<K>();   // row stride of the walk's W
             R acc[WK::HJ];
 #pragma unroll
             for (int j = 0; j < WK::HJ; j++) acc[j] = R(0);
@@ -1848,11 +1858,11 @@ __device__ __forceinline__ ParticleMove<R> lp_propose(const Walker<R, NP, K, RB>
 // ------------------------------------------------------------------------
 template <typename R, int K>
 struct RbmLds {
-    double* accW;    // [2][ND*H] sum over the wave's walkers of xs_k n_h, and of E xs_k n_h
+    double* accW;    // [2][ND][H + 1] sum over the wave's walkers of xs_k n_h, and of E xs_k n_h
     double* accAB;   // [2][ND + H] the a and b parts of the same sums
     double* E;       // [G] staged local energies
     R* W;            // [max(ND, 32)][4K], rows from ND on zero
-    R* X;            // [G][32] matrix-core results, one row per walker
+    R* X;            // [G][kMfX] matrix-core results, one row per walker
     R* xs;           // [G][2][ND] sorted coordinates: current, proposal
     R* n;            // [G][H]
     R* bc;           // [2][K][4]: b_h and sum_k W[k][h]^2, unit h = l + K j at [l][j]
@@ -1861,17 +1871,17 @@ struct RbmLds {
     {
         constexpr int G = 64 / K;
         accW = base;
-        accAB = accW + 2 * ND * H;
+        accAB = accW + 2 * ND * rbm_hst(H);
         E = accAB + 2 * (ND + H);
         R* p = reinterpret_cast<R*>(E + G);   // 16-byte aligned: ND and G are even
         if (W_elsewhere) {
             W = W_elsewhere;
         } else {
             W = p;
-            p += (ND > 32 ? ND : 32) * 4 * K;
+            p += (ND > 32 ? ND : 32) * rbm_wst<K>();
         }
         X = p;
-        xs = X + G * 32;
+        xs = X + G * kMfX;
         n = xs + G * 2 * ND;
         bc = n + G * H;
     }
@@ -1880,9 +1890,9 @@ struct RbmLds {
 inline size_t rbm_lds_bytes(int K, int ND, int H, size_t rsize, bool w_elsewhere = false)
 {
     const int G = 64 / K;
-    const size_t wsz = w_elsewhere ? 0 : (size_t)(ND > 32 ? ND : 32) * 4 * K;
-    return sizeof(double) * (2 * ND * H + 2 * (ND + H) + G) +
-           rsize * (wsz + (size_t)G * 32 + (size_t)G * 2 * ND + (size_t)G * H + (size_t)8 * K);
+    const size_t wsz = w_elsewhere ? 0 : (size_t)(ND > 32 ? ND : 32) * (4 * K + 1);
+    return sizeof(double) * (2 * ND * rbm_hst(H) + 2 * (ND + H) + G) +
+           rsize * (wsz + (size_t)G * kMfX + (size_t)G * 2 * ND + (size_t)G * H + (size_t)8 * K);
 }
 
 // LDS the walk of one block needs besides the dynamic RBM / learned-proposal
@@ -1896,7 +1906,7 @@ constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 5 * (64 / 
 template <typename R, int NP, int K>
 constexpr bool rbm_w_fits_static()
 {
-    return walk_static_lds_doubles<NP, K>() * sizeof(double) >= 32 * 4 * K * sizeof(R);
+    return walk_static_lds_doubles<NP, K>() * sizeof(double) >= 32 * rbm_wst<K>() * sizeof(R);
 }
 
 template <typename R, int NP, int K, bool RB>
@@ -1952,8 +1962,8 @@ __device__ __forceinline__ void rbm_mf_wgrad(const WalkParams& P, const RbmLds<f
             for (int i = 0; i < 4; i++) {
                 const int k = 16 * tk + 4 * q + i;
                 if (k < ND && hc < H) {
-                    L.accW[k * H + hc] += double(d1[i]);
-                    L.accW[ND * H + k * H + hc] += double(d2[i]);
+                    L.accW[k * rbm_hst(H) + hc] += double(d1[i]);
+                    L.accW[ND * rbm_hst(H) + k * rbm_hst(H) + hc] += double(d2[i]);
                 }
                 if (BT && k == ND && hc < H) {
                     L.accAB[ND + hc] += double(d1[i]);
@@ -2067,7 +2077,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     auto rbm_start = [&](R* W_elsewhere) {
         L.carve(rbm_dyn, P.ND, P.H, W_elsewhere);
         rbm_bind(w, L, P);
-        for (int i = lane; i < 2 * P.ND * P.H; i += 64) L.accW[i] = 0.0;
+        for (int i = lane; i < 2 * P.ND * rbm_hst(P.H); i += 64) L.accW[i] = 0.0;
         for (int i = lane; i < 2 * (P.ND + P.H); i += 64) L.accAB[i] = 0.0;
         w.rbm_setup(P);
         __syncthreads();
@@ -2211,8 +2221,8 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                             for (int hh = 0; hh < 4; hh++) {
                                 const int h = hb * hbs + hh;
                                 if (hh < hbs && h < P.H) {
-                                    L.accW[kc * P.H + h] += s1[hh];
-                                    L.accW[P.ND * P.H + kc * P.H + h] += s2[hh];
+                                    L.accW[kc * rbm_hst(P.H) + h] += s1[hh];
+                                    L.accW[P.ND * rbm_hst(P.H) + kc * rbm_hst(P.H) + h] += s2[hh];
                                 }
                             }
                         }
@@ -2276,7 +2286,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
                 v = L.accAB[hf * nab + r];
             } else {
                 const int m = r - nab, kc = m % P.ND, h = m / P.ND;
-                v = L.accW[hf * P.ND * P.H + kc * P.H + h] * wscale;
+                v = L.accW[hf * P.ND * rbm_hst(P.H) + kc * rbm_hst(P.H) + h] * wscale;
             }
             row[i] = D.acc_add ? row[i] + v : v;
         }
