@@ -1365,9 +1365,27 @@ __device__ __forceinline__ R lp_tanh(R a)
     }
 }
 
+// the network's weights in LDS: the parameter vector with the rows of W2 padded
+// from 64 to 72 bf16 (144 B), so that the sixteen rows a matrix-core operand
+// (or a wave's lanes) reads at one column do not share their banks; the other
+// blocks keep their order after it
+constexpr int kLpW2S = 72;
+constexpr int kLdsW2 = kLpW2;
+constexpr int kLdsB2 = kLdsW2 + kLpHid * kLpW2S;
+constexpr int kLdsW3 = kLdsB2 + kLpHid;
+constexpr int kLdsB3 = kLdsW3 + kLpOut * kLpHid;
+constexpr int kLdsParams = kLdsB3 + kLpOut;
+// LDS position of parameter i
+__device__ __forceinline__ int lp_lds_of(int i)
+{
+    if (i < kLpW2) return i;
+    if (i < kLpB2) return kLdsW2 + ((i - kLpW2) / kLpHid) * kLpW2S + (i - kLpW2) % kLpHid;
+    return i - kLpB2 + kLdsB2;
+}
+
 template <typename R, int K>
 struct LpLds {
-    uint16_t* W;   // [kLpParams]
+    uint16_t* W;   // [kLdsParams], see kLpW2S
     R* h;          // [G][kLpHid] first-layer activations of each walker
     float* xa;     // [G][kLpHid] matrix-core pre-activations (FP32 walks, K >= 16)
     uint32_t* fs;  // [G][16] features as bf16 pairs, zero from 18 on (FP32 walks, K >= 16)
@@ -1381,7 +1399,7 @@ constexpr bool lp_mf(size_t rsize, int K) { return rsize == 4 && K >= 16; }
 // bytes of the network's LDS (weights, activations, matrix-core staging)
 constexpr size_t lp_lds_bytes_c(int K, size_t rsize)
 {
-    const size_t base = rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLpParams;
+    const size_t base = rsize * (size_t)(64 / K) * kLpHid + sizeof(uint16_t) * (size_t)kLdsParams;
     return lp_mf(rsize, K) ? ((base + 15) & ~size_t(15)) + sizeof(float) * (64 / K) * kLpHid +
                                  sizeof(uint32_t) * (64 / K) * 16
                            : base;
@@ -1395,7 +1413,7 @@ __device__ __forceinline__ void lp_carve(LpLds<R, K>& L, double* base)
     L.h = reinterpret_cast<R*>(base);
     L.W = reinterpret_cast<uint16_t*>(L.h + G * kLpHid);
     if constexpr (lp_mf(sizeof(R), K)) {
-        const size_t off = ((sizeof(R) * G * kLpHid + sizeof(uint16_t) * kLpParams) + 15) & ~size_t(15);
+        const size_t off = ((sizeof(R) * G * kLpHid + sizeof(uint16_t) * kLdsParams) + 15) & ~size_t(15);
         L.xa = reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off);
         L.fs = reinterpret_cast<uint32_t*>(L.xa + G * kLpHid);
     } else {
@@ -1564,11 +1582,11 @@ __device__ __forceinline__ void lp_mlp_bf16(const LpLds<float, K>& L, int g, int
 #pragma unroll UNR
     for (int j = 0; j < UPL; j++) {
         const int u = l + K * j;
-        float a2 = L.w(kLpB2 + u);
+        float a2 = L.w(kLdsB2 + u);
 #pragma unroll
         for (int c = 0; c < kLpHid; c += 8) {
             const uint4 h = *reinterpret_cast<const uint4*>(hb + c);
-            const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLpW2 + u * kLpHid + c);
+            const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLdsW2 + u * kLpW2S + c);
             a2 = lp_dot2(v.x, h.x, a2);
             a2 = lp_dot2(v.y, h.y, a2);
             a2 = lp_dot2(v.z, h.z, a2);
@@ -1576,11 +1594,11 @@ __device__ __forceinline__ void lp_mlp_bf16(const LpLds<float, K>& L, int g, int
         }
         const float h2 = lp_tanh(a2);
 #pragma unroll
-        for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLpW3 + r * kLpHid + u) * h2;
+        for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLdsW3 + r * kLpHid + u) * h2;
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLpB3 + r);
+    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLdsB3 + r);
 }
 
 // FP32 walks, K >= 16: the hidden layers as BF16 matrix-core tiles (FP32
@@ -1654,7 +1672,7 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
     const uint4 b1 = keep(*reinterpret_cast<const uint4*>(hb + wv * kLpHid + 32 + 8 * q));
 #pragma unroll
     for (int tu = 0; tu < 4; tu++) {
-        const uint16_t* row = L.W + kLpW2 + (16 * tu + c) * kLpHid;
+        const uint16_t* row = L.W + kLdsW2 + (16 * tu + c) * kLpW2S;
         lp_f4 d = lp_tile(*reinterpret_cast<const uint4*>(row + 8 * q), b0, (lp_f4){0.f, 0.f, 0.f, 0.f});
         d = lp_tile(*reinterpret_cast<const uint4*>(row + 32 + 8 * q), b1, d);
         if (c < G) *reinterpret_cast<lp_f4*>(L.xa + c * kLpHid + 16 * tu + 4 * q) = d;
@@ -1664,13 +1682,13 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
 #pragma unroll
     for (int j = 0; j < UPL; j++) {
         const int u = l + K * j;
-        const float h2 = lp_tanh(L.xa[g * kLpHid + u] + L.w(kLpB2 + u));
+        const float h2 = lp_tanh(L.xa[g * kLpHid + u] + L.w(kLdsB2 + u));
 #pragma unroll
-        for (int r = 0; r < kLpOut; r++) tt[r] += L.w(kLpW3 + r * kLpHid + u) * h2;
+        for (int r = 0; r < kLpOut; r++) tt[r] += L.w(kLdsW3 + r * kLpHid + u) * h2;
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(tt[r]) + L.w(kLpB3 + r);
+    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(tt[r]) + L.w(kLdsB3 + r);
 }
 
 // FP32 walks on matrix-core tiles (lp_mf): the features of electron i at
@@ -1743,7 +1761,7 @@ __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const
         // c-blocks of 8: the activations once, then each unit's 8 weights in one read
         R a2[UPL];
 #pragma unroll
-        for (int j = 0; j < UPL; j++) a2[j] = L.w(kLpB2 + l + K * j);
+        for (int j = 0; j < UPL; j++) a2[j] = L.w(kLdsB2 + l + K * j);
         for (int c = 0; c < kLpHid; c += 8) {
             R h[8];
 #pragma unroll
@@ -1751,7 +1769,7 @@ __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const
 #pragma unroll
             for (int j = 0; j < UPL; j++) {
                 const int u = l + K * j;
-                const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLpW2 + u * kLpHid + c);
+                const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLdsW2 + u * kLpW2S + c);
                 a2[j] += R(bf16_lo(v.x)) * h[0] + R(bf16_hi(v.x)) * h[1] + R(bf16_lo(v.y)) * h[2] +
                          R(bf16_hi(v.y)) * h[3] + R(bf16_lo(v.z)) * h[4] + R(bf16_hi(v.z)) * h[5] +
                          R(bf16_lo(v.w)) * h[6] + R(bf16_hi(v.w)) * h[7];
@@ -1761,15 +1779,15 @@ __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const
         for (int j = 0; j < UPL; j++) {
             const R h2 = lp_tanh(a2[j]);
 #pragma unroll
-            for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLpW3 + r * kLpHid + l + K * j) * h2;
+            for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLdsW3 + r * kLpHid + l + K * j) * h2;
         }
     } else {
         // many units per lane: one unit at a time, the head folded in
         for (int j = 0; j < UPL; j++) {
             const int u = l + K * j;
-            R a2 = L.w(kLpB2 + u);
+            R a2 = L.w(kLdsB2 + u);
             for (int c = 0; c < kLpHid; c += 8) {
-                const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLpW2 + u * kLpHid + c);
+                const uint4 v = *reinterpret_cast<const uint4*>(L.W + kLdsW2 + u * kLpW2S + c);
                 a2 += R(bf16_lo(v.x)) * hb[c] + R(bf16_hi(v.x)) * hb[c + 1] +
                       R(bf16_lo(v.y)) * hb[c + 2] + R(bf16_hi(v.y)) * hb[c + 3] +
                       R(bf16_lo(v.z)) * hb[c + 4] + R(bf16_hi(v.z)) * hb[c + 5] +
@@ -1777,12 +1795,12 @@ __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const
             }
             const R h2 = lp_tanh(a2);
 #pragma unroll
-            for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLpW3 + r * kLpHid + u) * h2;
+            for (int r = 0; r < kLpOut; r++) t[r] += L.w(kLdsW3 + r * kLpHid + u) * h2;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLpB3 + r);
+    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLdsB3 + r);
 }
 
 // move mean and scale: mu = l_mu tanh(o/l_mu) / sqrt(omega), scale =
@@ -2067,7 +2085,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     constexpr bool lp_static = lp_fits_static<R, NP, K>();
     if constexpr (LP && !lp_static) {
         lp_carve(LL, rbm_dyn);
-        for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
+        for (int i = lane; i < kLpParams; i += 64) LL.W[lp_lds_of(i)] = P.lp_w[i];
         if constexpr (lp_mf(sizeof(R), K))   // the feature rows' zero padding
             for (int i = lane; i < G * 16; i += 64) LL.fs[i] = 0u;
         __syncthreads();
@@ -2094,7 +2112,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     if constexpr (LP && lp_static) {
         __syncthreads();   // the rebuild's reads of lds_T are done
         lp_carve(LL, lds_T);
-        for (int i = lane; i < kLpParams; i += 64) LL.W[i] = P.lp_w[i];
+        for (int i = lane; i < kLpParams; i += 64) LL.W[lp_lds_of(i)] = P.lp_w[i];
         if constexpr (lp_mf(sizeof(R), K))   // the feature rows' zero padding
             for (int i = lane; i < G * 16; i += 64) LL.fs[i] = 0u;
         __syncthreads();
