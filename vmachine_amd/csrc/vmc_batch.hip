@@ -51,7 +51,7 @@ __global__ __launch_bounds__(64) void batch_walk_kernel(const WalkParams P, int 
     D.window_lo = S * kb;
     D.window_hi = S * (kb + 1);
     D.init_qf = st->fresh;
-    walk_block<R, NP, K, false, false>(P, D, blockIdx.x, lds, lds + G * NP * NP, nullptr);
+    walk_block<R, NP, K, false, false>(P, D, blockIdx.x, lds, lds + walk_t_doubles<NP, K>(), nullptr);
 }
 
 // ---- one persistent launch per iteration ----
@@ -126,7 +126,7 @@ __device__ __attribute__((noinline)) void batch_tile(const BatchTable* table, in
         D.t_begin = t0;
         D.t_end = t0 + seg < total ? t0 + seg : total;
         D.acc_add = t0 > 0 ? 1 : 0;
-        walk_block<float, NP, K, false, false>(P, D, tile, lds, lds + G * NP * NP, nullptr);
+        walk_block<float, NP, K, false, false>(P, D, tile, lds, lds + walk_t_doubles<NP, K>(), nullptr);
     }
 }
 

@@ -557,17 +557,19 @@ struct Walker {
             }
         });
         // lane piv[c] now holds row c of the inverse with its columns permuted:
-        // a[j] = inv[c][piv[j]].  Transpose through LDS so lane q gets inv[.][q].
-        double* T = lds_T + g * NP * NP;
+        // a[j] = inv[c][piv[j]].  Transpose through LDS so lane q gets inv[.][q]
+        // (rows of NP + 1: the lanes' reads of one column fall into different banks)
+        constexpr int TS = NP + 1;
+        double* T = lds_T + g * NP * TS;
         auto transpose = [&](int b, R(&out)[NP]) {
             if (l < NP) {
 #pragma unroll
                 for (int j = 0; j < NP; j++)
-                    T[piv[((b0 + b) * G + g) * NP + j] * NP + mycol[b]] = a[b][j];
+                    T[piv[((b0 + b) * G + g) * NP + j] * TS + mycol[b]] = a[b][j];
             }
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < NP; k++) out[k] = l < NP ? R(T[l * NP + k]) : R(0);
+            for (int k = 0; k < NP; k++) out[k] = l < NP ? R(T[l * TS + k]) : R(0);
             __syncthreads();
         };
         transpose(0, out0);
@@ -1424,6 +1426,9 @@ __device__ __forceinline__ void lp_carve(LpLds<R, K>& L, double* base)
 
 template <int NP, int K>
 constexpr int walk_static_lds_doubles();
+// the rebuild's transpose area: per walker NP rows of NP + 1 doubles
+template <int NP, int K>
+constexpr int walk_t_doubles() { return (64 / K) * NP * (NP + 1); }
 
 // the learned proposal's LDS fits in the Slater rebuild's static LDS, which is
 // free once the rebuild is done (then a launch needs no dynamic LDS: 14.8 KB
@@ -1916,7 +1921,7 @@ inline size_t rbm_lds_bytes(int K, int ND, int H, size_t rsize, bool w_elsewhere
 // LDS the walk of one block needs besides the dynamic RBM / learned-proposal
 // area: the Gauss-Jordan transpose and pivot rows of the Slater rebuild
 template <int NP, int K>
-constexpr int walk_static_lds_doubles() { return (64 / K) * NP * NP + 5 * (64 / K) * NP; }
+constexpr int walk_static_lds_doubles() { return walk_t_doubles<NP, K>() + 5 * (64 / K) * NP; }
 
 // the walk's copy of the RBM-product W (32 rows: N D <= 32) fits in the Slater
 // rebuild's static LDS, free once the launch's rebuild is done (N=12 FP32:
@@ -2375,7 +2380,7 @@ void walk_kernel(const WalkParams P)
     // the RBM x Slater x Pade-Jastrow instantiations on tiles take the b sums
     // from the tiles (their launcher checks ND % 16 != 0)
     constexpr bool BT = RB && MF != 0 && SG >= 0 && sizeof(R) == 4;
-    walk_block<R, NP, K, RB, LP, DN, BT>(P, D, blockIdx.x, lds, lds + (64 / K) * NP * NP, rbm_dyn);
+    walk_block<R, NP, K, RB, LP, DN, BT>(P, D, blockIdx.x, lds, lds + walk_t_doubles<NP, K>(), rbm_dyn);
 }
 
 // ------------------------------------------------------------------------
@@ -2424,7 +2429,7 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
 {
     using WK = Walker<R, NP, K, RB>;
     constexpr int G = WK::G;
-    __shared__ double lds_T[G * NP * NP];
+    __shared__ double lds_T[walk_t_doubles<NP, K>()];
     __shared__ double lds_row[5 * G * NP];
     extern __shared__ double rbm_dyn[];
     WK w;
