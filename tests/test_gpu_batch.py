@@ -344,8 +344,9 @@ def test_c4_mixed_batch_at_full_size():
     batch from the same walkers and streams gives per-system energies within
     4 standard errors of the FP32 batch's (the error of a mean over W walkers
     is at most sigma / sqrt(W) whatever the autocorrelation within each
-    walker's 200 steps).  (c) After the autotuner, optimisation iterations
-    with ADAM replayed as a hipGraph equal direct launches bit for bit.  No
+    walker's 200 steps).  (c) After the autotuner (shapes and launch form),
+    optimisation iterations with ADAM replayed as a hipGraph equal direct
+    launches with the same shapes and launch form bit for bit.  No
     sampled local energy is non-finite."""
     V = _require_gpu()
     systems = _c4_systems()
@@ -397,10 +398,12 @@ def test_c4_mixed_batch_at_full_size():
             b.init_walkers(seed)
             if lanes is None:
                 lanes, ns = b.autotune(30)
-                print(f"  autotune lanes {lanes.tolist()}", flush=True)
+                launch = b.launch()
+                print(f"  autotune lanes {lanes.tolist()} launch {launch}", flush=True)
             else:
                 for s in range(len(systems)):
                     b.set_shape(s, int(lanes[s]))
+                b.set_launch(launch)
             out, ms = b.run(3, C4_STEPS, 0, learning_rate=0.01, use_graph=use_graph)
             runs.append((out, [b.get_positions(s) for s in range(len(systems))], b.parameters()))
             print(f"  graph={use_graph}: {ms:.1f} ms for 3 iterations", flush=True)
