@@ -588,7 +588,7 @@ def main():
     if args.workload == "batch":
         batch_workload(args, V)
         return
-    n = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28}[args.N]
+    n = {2: 1, 6: 3, 12: 6, 20: 10, 30: 15, 42: 21, 56: 28, 72: 36, 90: 45, 110: 55}[args.N]
     pos = equilibrate(V, args)
 
     # ---- FP32 walk, exact Langevin dt (value) ----

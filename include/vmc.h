@@ -48,7 +48,9 @@ extern "C" {
      VMC_Q13_RBMG_PSI_RATIO)
 
 typedef struct vmc_config {
-    int num_particles;      /* numParticles */
+    int num_particles;      /* numParticles: 1..110 (with a Slater determinant the closed
+                               shells 2, 6, 12, 20, 30, 42, 56, 72, 90, 110; the learned
+                               proposal and batches up to 56) */
     int num_dimensions;     /* numDimensions (the engine supports 2) */
     double omega;           /* omega */
     int interaction;        /* interactionStyle: 0 noInteraction, 1 coulomb */

@@ -231,3 +231,13 @@ def test_fp32_learned_walk_samples_psi_squared(N, omega):
         print(f"  {label}: E {res[label][0]:.4f} +- {res[label][1]:.4f} tau {res[label][2]:.1f}")
     (El, el, _), (Eg, eg, _) = res["learned"], res["langevin"]
     assert abs(El - Eg) < 4 * np.hypot(el, eg)
+
+
+def test_learned_proposal_limited_to_56_particles():
+    """The learned proposal's kernels exist for the closed shells up to N = 56;
+    the engine refuses it past them at creation."""
+    import vmachine_amd as V
+    cfg = V.make_config(72, omega=0.1, kinds=SGP, sampling=V.LEARNED, step_length=0.01, quirks=0,
+                        precision=32, num_walkers=8)
+    with pytest.raises(V.VmcError, match="learned proposal"):
+        V.Engine(cfg)

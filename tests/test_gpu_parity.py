@@ -50,7 +50,12 @@ def test_evaluate_matches_oracle(N, precision):
     errs = scaled_errors(g0, r0, g1, r1)
     print(f"N={N} precision={precision} " +
           " ".join(f"{k}: max {v.max():.2e} median {np.median(v):.2e}" for k, v in errs.items()))
-    tol_max, tol_med = TOL[precision]
+    # 72 to 110 particles at independent random positions: the 36..55-row
+    # Slater matrices of some walkers are conditioned to 1e8 and more, and the
+    # drift (the gradient of ln det) carries that on both sides, so the bounds
+    # are looser there; the FP64 chains at N = 72 and 110 (typical
+    # configurations) still retrace the oracle to 1e-9
+    tol_max, tol_med = (TOL if N <= 56 else TOL_LARGE)[precision]
     for k, v in errs.items():
         assert v.max() < tol_max, f"{k}: max scaled error {v.max():.3e} >= {tol_max}"
         assert np.median(v) < tol_med, f"{k}: median scaled error {np.median(v):.3e} >= {tol_med}"
@@ -58,6 +63,7 @@ def test_evaluate_matches_oracle(N, precision):
 
 # scaled-error bounds (max over walkers, median over walkers), see scaled_errors()
 TOL = {64: (1e-9, 1e-11), 32: (2e-3, 2e-5)}
+TOL_LARGE = {64: (2e-7, 5e-10), 32: (5e-2, 2e-4)}
 
 
 def scaled_errors(g0, r0, g1, r1):
@@ -93,6 +99,8 @@ def scaled_errors(g0, r0, g1, r1):
     (20, O.IMPORTANCE_COMPAT, 0.01, KINDS_SGP, 0.0),
     (30, O.IMPORTANCE_EXACT, 0.01, KINDS_SGP, 1.0),
     (56, O.IMPORTANCE_EXACT, 0.005, KINDS_SGP, 1.0),
+    (72, O.IMPORTANCE_EXACT, 0.005, KINDS_SGP, 1.0),
+    (110, O.IMPORTANCE_COMPAT, 0.005, KINDS_SGP, 0.0),
 ])
 def test_fp64_trajectories_match_oracle(N, sampling, dt, kinds, drift):
     """Same Philox streams, same positions: the FP64 walk must retrace the

@@ -23,6 +23,16 @@
    (its own stationary variance is dt / (1 - (1 - alpha omega dt)^2), 21 %
    above the target at alpha = 0.7, omega = 1), so a dropped or sign-flipped
    Green's-function ratio misses both closed forms by many standard errors.
+
+3. The oracle's scaled basis (basis_scaled: the closed-shell orbitals as
+   H_nx(sqrt(omega) x) H_ny(sqrt(omega) y) instead of the reference's
+   H_nx(x) H_ny(y), ref src/Basis/hermite.cpp:33-71).  Both sets span the same
+   polynomials of degree <= n per shell, so the Slater matrices differ by an
+   invertible shell-triangular factor and every determinant ratio, local
+   energy, drift and parameter gradient is the same.  Checked on random
+   configurations at the closed shells up to N = 42, where the reference's
+   basis is still well conditioned in FP64; the oracle uses the scaled basis
+   past N = 56, where it is not.
 """
 import numpy as np
 import pytest
@@ -81,3 +91,29 @@ def test_exact_langevin_step_samples_psi_squared(alpha, omega, dt):
     r2_proposal = 4.0 * dt / (1.0 - q * q)
     assert abs(r2_proposal - r2_exact) > 20.0 * se_r2
     assert 0.3 < acc.mean() < 0.99
+
+
+@pytest.mark.parametrize("N,omega", [(2, 1.0), (6, 1.0), (12, 0.5), (20, 0.5), (30, 0.28),
+                                     (42, 0.28)])
+def test_scaled_basis_gives_the_reference_basis_values(N, omega):
+    from tests.helpers import oracle_evaluate, random_parameters
+    kinds = [O.SLATER, O.GAUSSIAN, O.PADE]
+    rng = np.random.default_rng(500 + N)
+    W = 12
+    P = random_parameters(rng, kinds)
+    pos = rng.standard_normal((W, 2 * N)) / np.sqrt(omega)
+    moves = rng.integers(0, 2 * N, W)
+    newv = pos[np.arange(W), moves] + 0.3 / np.sqrt(omega) * rng.standard_normal(W)
+    res = []
+    for scaled in (False, True):
+        cfg = O.make_config(N, 2, omega, 1, kinds, None, None, False, O.IMPORTANCE_EXACT, 0.01,
+                            quirks=O.Q2 | O.Q3, basis_scaled=scaled)
+        res.append(oracle_evaluate(cfg, P, pos, moves, newv))
+    for a, b in zip(res[0], res[1]):
+        scale = np.maximum(np.abs(a), 1.0)
+        assert np.max(np.abs(a - b) / scale) < 1e-8, np.max(np.abs(a - b) / scale)
+    # not vacuous: the same comparison sees a 10 % change of the y coordinates
+    cfg = O.make_config(N, 2, omega, 1, kinds, None, None, False, O.IMPORTANCE_EXACT, 0.01,
+                        quirks=O.Q2 | O.Q3, basis_scaled=True)
+    moved = oracle_evaluate(cfg, P, pos * np.array([1.0, 0.9] * N), moves, newv)
+    assert np.max(np.abs(moved[0][:, 0] - res[0][0][:, 0])) > 1e-3

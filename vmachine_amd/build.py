@@ -30,11 +30,14 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fno-signe
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
-# the walk kernels are split by precision so the two halves compile in parallel
+# the walk kernels are split by precision (and the closed shells past N = 56
+# into units of their own) so that they compile in parallel
 UNITS = [
     ("vmc_walk_f32.o", "vmc_walk.hip", ["-DVMC_WALK_PRECISION=32"]),
     ("vmc_walk_f64.o", "vmc_walk.hip", ["-DVMC_WALK_PRECISION=64"]),
     ("vmc_walk_common.o", "vmc_walk.hip", ["-DVMC_WALK_PRECISION=0"]),
+    ("vmc_walk_large_f32.o", "vmc_walk.hip", ["-DVMC_WALK_PRECISION=32", "-DVMC_WALK_LARGE"]),
+    ("vmc_walk_large_f64.o", "vmc_walk.hip", ["-DVMC_WALK_PRECISION=64", "-DVMC_WALK_LARGE"]),
     ("vmc_learn.o", "vmc_learn.hip", []),
     ("vmc_batch.o", "vmc_batch.hip", []),
     ("vmc_optim.o", "vmc_optim.hip", []),

@@ -83,9 +83,14 @@ int fail(int code, const std::string& msg)
             return fail(VMC_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-const int kNP[] = {1, 3, 6, 10, 15, 21, 28};
+// orbitals per spin of the closed shells N = 2, 6, ..., 110 (the last three,
+// N = 72, 90, 110, one walker per wave)
+const int kNP[] = {1, 3, 6, 10, 15, 21, 28, 36, 45, 55};
 constexpr int kRowChunks = 64;   // partial rows of launch_reduce_rows
-const int kLanes[] = {1, 4, 8, 16, 16, 32, 32};   // lanes per walker for each kNP entry
+const int kLanes[] = {1, 4, 8, 16, 16, 32, 32, 64, 64, 64};   // lanes per walker for each kNP entry
+constexpr int kMaxParticles = 110;
+constexpr int kLearnedMaxOrbitals = 28;   // the learned proposal: N <= 56
+constexpr int kBatchMaxOrbitals = 28;     // batches: N <= 56
 
 // RBM instantiations: (np, K) pairs compiled with the RBM elements
 const int kRbmShapes[][2] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {3, 4}, {3, 8}, {6, 8}, {10, 16}};
@@ -244,8 +249,8 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
     const vmc_config& c = *cfg;
     if (c.num_dimensions != 2)
         return fail(VMC_E_INVALID, "the engine supports numDimensions: 2");
-    if (c.num_particles < 1 || c.num_particles > 56)
-        return fail(VMC_E_INVALID, "numParticles must be in 1..56");
+    if (c.num_particles < 1 || c.num_particles > kMaxParticles)
+        return fail(VMC_E_INVALID, "numParticles must be in 1..110");
     if (c.num_elements < 1 || c.num_elements > 6)
         return fail(VMC_E_INVALID, "between 1 and 6 wave-function elements");
     for (int e = 0; e < c.num_elements; e++) {
@@ -281,7 +286,7 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
         np = closed_shell_orbitals(c.num_particles);
         if (np < 0)
             return fail(VMC_E_INVALID, "slaterDeterminant needs a closed-shell numParticles "
-                                       "(2, 6, 12, 20, 30, 42, 56)");
+                                       "(2, 6, 12, 20, 30, 42, 56, 72, 90, 110)");
     } else {
         const int half = (c.num_particles + 1) / 2;
         np = 0;
@@ -289,6 +294,8 @@ int vmc_engine_create(const vmc_config* cfg, vmc_engine** out)
             if (v >= half) { np = v; break; }
     }
 
+    if (c.sampling == VMC_LEARNED && np > kLearnedMaxOrbitals)
+        return fail(VMC_E_INVALID, "the learned proposal is available for numParticles <= 56");
     int K = 0;
     for (size_t i = 0; i < sizeof(kNP) / sizeof(kNP[0]); i++)
         if (kNP[i] == np) K = kLanes[i];
@@ -1374,7 +1381,9 @@ int vmc_batch_create(const vmc_batch_config* cfg, vmc_batch** out)
         int np = -1;
         if (has_slater) {
             np = closed_shell_orbitals(y.num_particles);
-            if (np < 0) return cleanup(fail(VMC_E_INVALID, tag + "slaterDeterminant needs a closed shell"));
+            if (np < 0 || np > kBatchMaxOrbitals)
+                return cleanup(fail(VMC_E_INVALID, tag + "slaterDeterminant needs a closed shell "
+                                                         "of at most 56 particles in a batch"));
         } else if (y.num_particles >= 1 && y.num_particles <= 56) {
             const int half = (y.num_particles + 1) / 2;
             for (int v : kNP)

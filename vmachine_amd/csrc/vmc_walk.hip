@@ -47,16 +47,18 @@ constexpr double csqrt_iter(double x, double cur, double prev)
 }
 constexpr double csqrt(double x) { return x <= 0 ? 0.0 : csqrt_iter(x, x > 1 ? x : 1.0, 0.0); }
 
+// degrees 0..17, as the reference's table (ref src/Basis/hermite.cpp:342-386)
+constexpr int kHermiteN = 18;
 struct HermiteConst {
-    double c1[8];  // sqrt(2/(m+1))          h_{m+1} = c1 t h_m - c2 h_{m-1}
-    double c2[8];  // sqrt(m/(m+1))
-    double d1[8];  // sqrt(2m):      h_m'  = d1 h_{m-1}
-    double d2[8];  // 2 sqrt(m(m-1)): h_m'' = d2 h_{m-2}
+    double c1[kHermiteN];  // sqrt(2/(m+1))          h_{m+1} = c1 t h_m - c2 h_{m-1}
+    double c2[kHermiteN];  // sqrt(m/(m+1))
+    double d1[kHermiteN];  // sqrt(2m):      h_m'  = d1 h_{m-1}
+    double d2[kHermiteN];  // 2 sqrt(m(m-1)): h_m'' = d2 h_{m-2}
 };
 constexpr HermiteConst make_hermite_const()
 {
     HermiteConst h{};
-    for (int m = 0; m < 8; m++) {
+    for (int m = 0; m < kHermiteN; m++) {
         h.c1[m] = csqrt(2.0 / (m + 1));
         h.c2[m] = csqrt(double(m) / (m + 1));
         h.d1[m] = csqrt(2.0 * m);
@@ -496,14 +498,16 @@ struct Walker {
             constexpr int c = decltype(ci)::value;
             // the pivot: the group maximum of one 32-bit key per lane, the high
             // word of |a| (sign, exponent and 20 mantissa bits, which order as
-            // the value) with the low 5 bits replaced by the lane, found with
-            // DPP steps; magnitudes within 2^-15 of each other count as equal
+            // the value) with the low 5 bits (6 for 64 lanes) replaced by the
+            // lane, found with DPP steps; magnitudes within 2^-15 (2^-14) of
+            // each other count as equal
+            constexpr uint32_t kLaneMask = K > 32 ? 63u : 31u;
             int idx[NBK];
 #pragma unroll
             for (int b = 0; b < NBK; b++) {
                 const uint32_t hiw = (uint32_t)(__builtin_bit_cast(uint64_t, fabs(a[b][c])) >> 32);
-                const uint32_t key = used[b] ? 0u : ((hiw & ~31u) | (uint32_t)l);
-                idx[b] = (int)(gmax32<K>(key) & 31u);
+                const uint32_t key = used[b] ? 0u : ((hiw & ~kLaneMask) | (uint32_t)l);
+                idx[b] = (int)(gmax32<K>(key) & kLaneMask);
             }
             if constexpr (NBK == 2) {
                 // both blocks' pivot rows leave in one set of (masked) LDS
@@ -578,7 +582,15 @@ struct Walker {
 
     __device__ __forceinline__ void rebuild_slater(const WalkParams& P, double* lds_T, double* lds_row)
     {
-        invert_blocks<2>(P, 0, lds_T, lds_row, a0, a1);
+        if constexpr (NP > 28) {
+            // one block at a time: two interleaved FP64 blocks of NP > 28 rows
+            // would not fit the registers
+            invert_blocks<1>(P, 0, lds_T, lds_row, a0, a0);
+            __syncthreads();   // the first transpose's reads are done
+            invert_blocks<1>(P, 1, lds_T, lds_row, a1, a1);
+        } else {
+            invert_blocks<2>(P, 0, lds_T, lds_row, a0, a1);
+        }
         act = 0;
         R gx, gy, sec;
         slater_derivs(P, x[0][0], x[0][1], a0, gx, gy, sec);
@@ -2571,14 +2583,25 @@ static hipError_t launch_eval_t(const WalkParams& P, const double* pos, const in
     return hipGetLastError();
 }
 
+// (np, K): Slater orbitals per spin (or particle slots per spin without a
+// determinant) and lanes per walker; rbm selects the RBM instantiations.  The
+// closed shells past N = 56 (N = 72, 90, 110: one walker per wave, 64 lanes)
+// are a translation unit of their own per precision (VMC_WALK_LARGE), without
+// RBM elements and the learned proposal.
+#ifdef VMC_WALK_LARGE
+#if VMC_WALK_PRECISION == 32
+#define VMC_SUFFIX(name) name##_large_f32
+#else
+#define VMC_SUFFIX(name) name##_large_f64
+#endif
+#define VMC_SHAPES(X) X(36, 64, false) X(45, 64, false) X(55, 64, false)
+#define VMC_LP_SHAPES(X)
+#else
 #if VMC_WALK_PRECISION == 32
 #define VMC_SUFFIX(name) name##_f32
 #else
 #define VMC_SUFFIX(name) name##_f64
 #endif
-
-// (np, K): Slater orbitals per spin (or particle slots per spin without a
-// determinant) and lanes per walker; rbm selects the RBM instantiations
 #define VMC_SHAPES(X)                                                        \
     X(1, 1, false) X(3, 4, false) X(6, 8, false) X(10, 16, false)            \
     X(15, 16, false) X(21, 32, false) X(28, 32, false)                       \
@@ -2587,6 +2610,7 @@ static hipError_t launch_eval_t(const WalkParams& P, const double* pos, const in
 
 // shapes with the learned proposal (no RBM elements)
 #define VMC_LP_SHAPES(X) X(1, 1) X(3, 4) X(6, 8) X(10, 16) X(15, 16) X(21, 32) X(28, 32)
+#endif
 
 // one shape's launch: density counting, then the specialised samplers of the
 // plain shapes, then the generic instantiation
@@ -2643,6 +2667,7 @@ hipError_t VMC_SUFFIX(launch_eval)(int np, int K, bool rbm, const WalkParams& P,
     return hipErrorInvalidValue;
 }
 
+#ifndef VMC_WALK_LARGE
 // the draws of n consecutive steps of one walker, in the walk precision
 template <typename R>
 __global__ void draws_kernel(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
@@ -2667,6 +2692,7 @@ hipError_t VMC_SUFFIX(launch_draws)(uint64_t seed, uint32_t walker, uint64_t cou
                        counter0, nd, n, k, z, um, ua);
     return hipGetLastError();
 }
+#endif
 #endif
 
 #if VMC_WALK_PRECISION == 0
@@ -2744,6 +2770,17 @@ hipError_t launch_eval_f64(int np, int K, bool rbm, const WalkParams& P, const d
                            const int* moves, const double* newv, const EvalOut& O,
                            hipStream_t stream);
 
+hipError_t launch_walk_large_f32(int np, int K, bool rbm, bool lp, const WalkParams& P,
+                                 hipStream_t stream);
+hipError_t launch_walk_large_f64(int np, int K, bool rbm, bool lp, const WalkParams& P,
+                                 hipStream_t stream);
+hipError_t launch_eval_large_f32(int np, int K, bool rbm, const WalkParams& P, const double* pos,
+                                 const int* moves, const double* newv, const EvalOut& O,
+                                 hipStream_t stream);
+hipError_t launch_eval_large_f64(int np, int K, bool rbm, const WalkParams& P, const double* pos,
+                                 const int* moves, const double* newv, const EvalOut& O,
+                                 hipStream_t stream);
+
 hipError_t launch_draws_f32(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
                             int* k, double* z, double* um, double* ua, hipStream_t stream);
 hipError_t launch_draws_f64(uint64_t seed, uint32_t walker, uint64_t counter0, int nd, int n,
@@ -2753,6 +2790,9 @@ hipError_t launch_walk(int precision, const KernelShape& sh, const WalkParams& P
                        hipStream_t stream)
 {
     const bool lp = P.sampling == kLearned;
+    if (sh.np > 28)   // the closed shells past N = 56
+        return precision == 32 ? launch_walk_large_f32(sh.np, sh.K, sh.rbm, lp, P, stream)
+                               : launch_walk_large_f64(sh.np, sh.K, sh.rbm, lp, P, stream);
     return precision == 32 ? launch_walk_f32(sh.np, sh.K, sh.rbm, lp, P, stream)
                            : launch_walk_f64(sh.np, sh.K, sh.rbm, lp, P, stream);
 }
@@ -2768,6 +2808,10 @@ hipError_t launch_eval(int precision, const KernelShape& sh, const WalkParams& P
                        const double* pos, const int* moves, const double* newv, const EvalOut& O,
                        hipStream_t stream)
 {
+    if (sh.np > 28)
+        return precision == 32
+                   ? launch_eval_large_f32(sh.np, sh.K, sh.rbm, P, pos, moves, newv, O, stream)
+                   : launch_eval_large_f64(sh.np, sh.K, sh.rbm, P, pos, moves, newv, O, stream);
     return precision == 32 ? launch_eval_f32(sh.np, sh.K, sh.rbm, P, pos, moves, newv, O, stream)
                            : launch_eval_f64(sh.np, sh.K, sh.rbm, P, pos, moves, newv, O, stream);
 }
