@@ -1708,9 +1708,9 @@ int vmc_batch_autotune(vmc_batch* b, int steps, int* lanes_out, double* ns_per_s
     if (!batch_persistent_ok(b)) return finish(VMC_OK);
     // the launch: the whole batch walked side by side and as one persistent
     // launch (the faster of the second and third run of each), from the same
-    // positions and drifts,
-    // over `steps` steps plus up to one refresh interval, so that the trial
-    // crosses a refresh boundary as an iteration does; the faster is kept
+    // positions and drifts, over `steps` steps plus up to one refresh
+    // interval, so that the trial crosses a refresh boundary as an iteration
+    // does; the faster form is kept
     (void)hipFree(d_save);
     d_save = nullptr;
     if ((!b->persistent_grid && (err = vmc::batch_persistent_grid(&b->persistent_grid)) != hipSuccess) ||
