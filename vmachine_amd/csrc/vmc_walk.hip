@@ -1829,10 +1829,16 @@ __device__ __forceinline__ void lp_head(const WalkParams& P, const R (&o)[kLpOut
 {
     const R ell = R(P.inv_sqrt_omega);
     const R mm = R(P.lp_mu_max), ls = R(P.lp_ls_max);
+    // FP32: the quotients by the two bounds as products with their
+    // reciprocals (an IEEE division is about ten instructions; both passes of
+    // a proposal evaluate this same function, so the Metropolis-Hastings
+    // ratio stays exact)
+    const bool f32 = std::is_same<R, float>::value;
+    const R imm = R(1.0 / P.lp_mu_max), ils = R(1.0 / P.lp_ls_max);
 #pragma unroll
     for (int d = 0; d < 2; d++) {
-        mu[d] = ell * mm * lp_tanh(o[d] / mm);
-        lt[d] = ls * lp_tanh(o[2 + d] / ls);
+        mu[d] = ell * mm * lp_tanh(f32 ? o[d] * imm : o[d] / mm);
+        lt[d] = ls * lp_tanh(f32 ? o[2 + d] * ils : o[2 + d] / ls);
         sg[d] = R(P.lp_sigma0) * ell * M<R>::exp(lt[d]);
     }
 }
