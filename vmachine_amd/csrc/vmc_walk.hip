@@ -1732,13 +1732,16 @@ __device__ __forceinline__ void lp_features_staged(const Walker<float, NP, K, RB
         key[sl] = (w.valid[sl] && q != i) ? ((__float_as_uint(d2) & ~63u) | (uint32_t)q) : ~0u;
         word[sl] = __builtin_bit_cast(uint32_t, lp_pack(s * e0, s * e1));
     }
+    // with NP >= 10 orbitals per spin (N >= 13) every electron has at least
+    // eight others, so every round finds one
+    constexpr bool always = NP >= 10;
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const uint32_t best = gmin32<K>(key[0] < key[1] ? key[0] : key[1]);
-        const bool found = best != ~0u;
+        const bool found = always || best != ~0u;
         const bool own0 = found && key[0] == best, own1 = found && key[1] == best;
         if (own0 || own1) fs[1 + m] = own0 ? word[0] : word[1];
-        else if (!found && w.l == 0) fs[1 + m] = 0u;
+        else if (!always && !found && w.l == 0) fs[1 + m] = 0u;
         key[0] = own0 ? ~0u : key[0];
         key[1] = own1 ? ~0u : key[1];
     }
