@@ -1379,12 +1379,17 @@ __device__ __forceinline__ R lp_tanh(R a)
     }
 }
 
-// the network's weights in LDS: the parameter vector with the rows of W2 padded
-// from 64 to 72 bf16 (144 B), so that the sixteen rows a matrix-core operand
-// (or a wave's lanes) reads at one column do not share their banks; the other
-// blocks keep their order after it
+// the network's weights in LDS: the parameter vector with the rows of W1
+// padded from 18 to 24 bf16 (48 B, zeros after the 18th: a lane's four-word
+// operand of the first layer is then one aligned load, no masked words) and the
+// rows of W2 from 64 to 72 bf16 (144 B, so that the sixteen rows a
+// matrix-core operand, or a wave's lanes, reads at one column do not share
+// their banks); the blocks keep their order
+constexpr int kLpW1S = 24;
 constexpr int kLpW2S = 72;
-constexpr int kLdsW2 = kLpW2;
+constexpr int kLdsW1 = 0;
+constexpr int kLdsB1 = kLdsW1 + kLpHid * kLpW1S;
+constexpr int kLdsW2 = kLdsB1 + kLpHid;
 constexpr int kLdsB2 = kLdsW2 + kLpHid * kLpW2S;
 constexpr int kLdsW3 = kLdsB2 + kLpHid;
 constexpr int kLdsB3 = kLdsW3 + kLpOut * kLpHid;
@@ -1392,9 +1397,17 @@ constexpr int kLdsParams = kLdsB3 + kLpOut;
 // LDS position of parameter i
 __device__ __forceinline__ int lp_lds_of(int i)
 {
-    if (i < kLpW2) return i;
+    if (i < kLpB1) return kLdsW1 + (i / kLpIn) * kLpW1S + i % kLpIn;
+    if (i < kLpW2) return i - kLpB1 + kLdsB1;
     if (i < kLpB2) return kLdsW2 + ((i - kLpW2) / kLpHid) * kLpW2S + (i - kLpW2) % kLpHid;
     return i - kLpB2 + kLdsB2;
+}
+// the weights into LDS (lanes of one wave), W1's row padding zeroed
+__device__ __forceinline__ void lp_load_weights(uint16_t* W, const uint16_t* w, int lane)
+{
+    for (int i = lane; i < kLpParams; i += 64) W[lp_lds_of(i)] = w[i];
+    constexpr int pad = kLpW1S - kLpIn;
+    for (int i = lane; i < kLpHid * pad; i += 64) W[kLdsW1 + (i / pad) * kLpW1S + kLpIn + i % pad] = 0;
 }
 
 template <typename R, int K>
@@ -1588,8 +1601,8 @@ __device__ __forceinline__ void lp_mlp_bf16(const LpLds<float, K>& L, int g, int
 #pragma unroll UNR
     for (int j = 0; j < UPL; j++) {
         const int u = l + K * j;
-        float a = L.w(kLpB1 + u);
-        const uint32_t* row = W32 + (kLpW1 + u * kLpIn) / 2;
+        float a = L.w(kLdsB1 + u);
+        const uint32_t* row = W32 + (kLdsW1 + u * kLpW1S) / 2;
 #pragma unroll
         for (int m = 0; m < kLpIn / 2; m++) a = lp_dot2(row[m], fp[m], a);
         hb[u] = __builtin_bit_cast(uint16_t, (__bf16)lp_tanh(a));
@@ -1665,14 +1678,14 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
         return make_uint4(k ? v.x : 0u, k ? v.y : 0u, k ? v.z : 0u, k ? v.w : 0u);
     };
     const uint4 b = keep(*reinterpret_cast<const uint4*>(L.fs + wv * 16 + 4 * q));   // features 8q .. 8q+7
-    // layer 1: W1 rows are 18 bf16 (36 B): four 4-byte words from index 8q,
-    // the ones past the 18th feature zero (q = 2: one word, q = 3: none)
+    // layer 1: the four words from index 8q of W1's 48-byte row; past the
+    // 18th feature they are the row's zero padding (q = 2) or the next row's
+    // first words (q = 3, the last row: the first-layer biases), finite
+    // values the features' zero padding multiplies
 #pragma unroll
     for (int tu = 0; tu < 4; tu++) {
         const int u = 16 * tu + c;
-        const uint32_t* r = W32 + (kLpW1 + u * kLpIn) / 2 + 4 * (q < 3 ? q : 2);
-        const uint4 a = make_uint4(q < 3 ? r[0] : 0u, q < 2 ? r[1] : 0u, q < 2 ? r[2] : 0u,
-                                   q < 2 ? r[3] : 0u);
+        const uint4 a = *reinterpret_cast<const uint4*>(W32 + (kLdsW1 + u * kLpW1S) / 2 + 4 * q);
         const lp_f4 d = lp_tile(a, b, (lp_f4){0.f, 0.f, 0.f, 0.f});
         if (c < G) *reinterpret_cast<lp_f4*>(L.xa + c * kLpHid + 16 * tu + 4 * q) = d;
     }
@@ -1680,7 +1693,7 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
 #pragma unroll
     for (int j = 0; j < UPL; j++) {
         const int u = l + K * j;
-        const float a = L.xa[g * kLpHid + u] + L.w(kLpB1 + u);
+        const float a = L.xa[g * kLpHid + u] + L.w(kLdsB1 + u);
         hb[g * kLpHid + u] = __builtin_bit_cast(uint16_t, (__bf16)lp_tanh(a));
     }
     __syncthreads();
@@ -1765,8 +1778,8 @@ __device__ __forceinline__ void lp_mlp(const LpLds<R, K>& L, int g, int l, const
 #pragma unroll UNR
     for (int j = 0; j < UPL; j++) {
         const int u = l + K * j;
-        R a = L.w(kLpB1 + u);
-        const uint32_t* row = W32 + (kLpW1 + u * kLpIn) / 2;   // 18 weights = 9 words, 4-byte aligned
+        R a = L.w(kLdsB1 + u);
+        const uint32_t* row = W32 + (kLdsW1 + u * kLpW1S) / 2;   // 18 weights = 9 words, 4-byte aligned
 #pragma unroll
         for (int m = 0; m < kLpIn / 2; m++) {
             const uint32_t v = row[m];
@@ -2111,7 +2124,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     constexpr bool lp_static = lp_fits_static<R, NP, K>();
     if constexpr (LP && !lp_static) {
         lp_carve(LL, rbm_dyn);
-        for (int i = lane; i < kLpParams; i += 64) LL.W[lp_lds_of(i)] = P.lp_w[i];
+        lp_load_weights(LL.W, P.lp_w, lane);
         if constexpr (lp_mf(sizeof(R), K))   // the feature rows' zero padding
             for (int i = lane; i < G * 16; i += 64) LL.fs[i] = 0u;
         __syncthreads();
@@ -2138,7 +2151,7 @@ __device__ __forceinline__ void walk_block(const WalkParams& P, const WalkDyn& D
     if constexpr (LP && lp_static) {
         __syncthreads();   // the rebuild's reads of lds_T are done
         lp_carve(LL, lds_T);
-        for (int i = lane; i < kLpParams; i += 64) LL.W[lp_lds_of(i)] = P.lp_w[i];
+        lp_load_weights(LL.W, P.lp_w, lane);
         if constexpr (lp_mf(sizeof(R), K))   // the feature rows' zero padding
             for (int i = lane; i < G * 16; i += 64) LL.fs[i] = 0u;
         __syncthreads();
@@ -2384,8 +2397,8 @@ void walk_kernel(const WalkParams P)
         __builtin_assume(P.has_slater == 1 && P.has_rbmg == 1 && P.has_rbmp == 1 && P.has_pade == 1);
         __builtin_assume(P.coulomb == 1 && P.has_gauss == 0);
     }
-    __shared__ double lds[walk_static_lds_doubles<NP, K>()];
-    extern __shared__ double rbm_dyn[];
+    __shared__ __align__(16) double lds[walk_static_lds_doubles<NP, K>()];
+    extern __shared__ __align__(16) double rbm_dyn[];
     WalkDyn D = walk_dyn(P);
     if (P.dyn_st) {   // an optimisation loop on the device: this iteration's values
         const OptState& st = *P.dyn_st;
@@ -2450,9 +2463,9 @@ __global__ __launch_bounds__(64) void eval_kernel(const WalkParams P, const doub
 {
     using WK = Walker<R, NP, K, RB>;
     constexpr int G = WK::G;
-    __shared__ double lds_T[walk_t_doubles<NP, K>()];
+    __shared__ __align__(16) double lds_T[walk_t_doubles<NP, K>()];
     __shared__ double lds_row[5 * G * NP];
-    extern __shared__ double rbm_dyn[];
+    extern __shared__ __align__(16) double rbm_dyn[];
     WK w;
     w.th_alpha = P.alpha;
     w.th_gamma = P.gamma;
