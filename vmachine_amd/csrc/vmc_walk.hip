@@ -1374,7 +1374,8 @@ __device__ __forceinline__ R lp_tanh(R a)
     if constexpr (std::is_same<R, double>::value) {
         return ::tanh(a);
     } else {
-        const R e = M<R>::exp(R(2) * a);
+        // e^(2a) as one base-2 exponential of a scaled by 2 log2(e)
+        const R e = __builtin_amdgcn_exp2f(a * R(2.0 * 1.4426950408889634));
         return R(1) - R(2) * M<R>::rcp(e + R(1));
     }
 }
@@ -1631,16 +1632,17 @@ __device__ __forceinline__ void lp_mlp_bf16(const LpLds<float, K>& L, int g, int
     for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(t[r]) + L.w(kLdsB3 + r);
 }
 
-// FP32 walks, K >= 16: the hidden layers as BF16 matrix-core tiles (FP32
+// FP32 walks, K >= 16: the three layers as BF16 matrix-core tiles (FP32
 // accumulation), the G <= 4 walkers of the wave the columns of 16 x 16 tiles
 // (V_MFMA_F32_16X16X32_BF16: result element r of lane t is unit 4q + r of
-// the tile, q = t / 16, column t % 16), 4 tiles of 16 units per layer; layer 1
-// contracts the 18 features (zero-padded to 32), layer 2 the 64 units in two
-// steps of 32.  Biases, tanh and the 4 outputs stay on the walker's lanes
+// the tile, q = t / 16, column t % 16), 4 tiles of 16 units per hidden layer;
+// layer 1 contracts the 18 features (zero-padded to 32), layers 2 and 3 the
+// 64 units in two steps of 32, layer 3 on one tile whose rows 0..3 are the
+// outputs.  The hidden layers' biases and tanh stay on the walker's lanes
 // (unit l + K j), reading the tiles' pre-activations through LDS.  Features
-// and activations are rounded to bfloat16 as in the dot-product form, and
-// both passes of a proposal evaluate this same function, so the
-// Metropolis-Hastings ratio stays exact.
+// and both layers' activations are rounded to bfloat16, and both passes of a
+// proposal evaluate this same function, so the Metropolis-Hastings ratio
+// stays exact.
 typedef __bf16 lp_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float lp_f4 __attribute__((ext_vector_type(4)));
 
@@ -1672,12 +1674,9 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
             *reinterpret_cast<uint4*>(L.fs + g * 16 + m) = make_uint4(p[m], p[m + 1], p[m + 2], p[m + 3]);
     }
     __syncthreads();
-    // component-wise selects (a select of whole uint4 values goes through scratch)
-    auto keep = [&](uint4 v) {
-        const bool k = c < G;
-        return make_uint4(k ? v.x : 0u, k ? v.y : 0u, k ? v.z : 0u, k ? v.w : 0u);
-    };
-    const uint4 b = keep(*reinterpret_cast<const uint4*>(L.fs + wv * 16 + 4 * q));   // features 8q .. 8q+7
+    // column c of a tile's result depends on column c of B only: the columns
+    // past the wave's walkers repeat walker G - 1 and are not read
+    const uint4 b = *reinterpret_cast<const uint4*>(L.fs + wv * 16 + 4 * q);   // features 8q .. 8q+7
     // layer 1: the four words from index 8q of W1's 48-byte row; past the
     // 18th feature they are the row's zero padding (q = 2) or the next row's
     // first words (q = 3, the last row: the first-layer biases), finite
@@ -1698,8 +1697,8 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
     }
     __syncthreads();
     // layer 2: B = the walker's first-layer activations, two steps of 32 units
-    const uint4 b0 = keep(*reinterpret_cast<const uint4*>(hb + wv * kLpHid + 8 * q));
-    const uint4 b1 = keep(*reinterpret_cast<const uint4*>(hb + wv * kLpHid + 32 + 8 * q));
+    const uint4 b0 = *reinterpret_cast<const uint4*>(hb + wv * kLpHid + 8 * q);
+    const uint4 b1 = *reinterpret_cast<const uint4*>(hb + wv * kLpHid + 32 + 8 * q);
 #pragma unroll
     for (int tu = 0; tu < 4; tu++) {
         const uint16_t* row = L.W + kLdsW2 + (16 * tu + c) * kLpW2S;
@@ -1708,17 +1707,25 @@ __device__ __forceinline__ void lp_mlp_mfma(const LpLds<float, K>& L, int g, int
         if (c < G) *reinterpret_cast<lp_f4*>(L.xa + c * kLpHid + 16 * tu + 4 * q) = d;
     }
     __syncthreads();
-    float tt[kLpOut] = {0.f, 0.f, 0.f, 0.f};
+    // second-layer activations, rounded to bf16, over the first ones (read)
 #pragma unroll
     for (int j = 0; j < UPL; j++) {
         const int u = l + K * j;
-        const float h2 = lp_tanh(L.xa[g * kLpHid + u] + L.w(kLdsB2 + u));
-#pragma unroll
-        for (int r = 0; r < kLpOut; r++) tt[r] += L.w(kLdsW3 + r * kLpHid + u) * h2;
+        const float a = L.xa[g * kLpHid + u] + L.w(kLdsB2 + u);
+        hb[g * kLpHid + u] = __builtin_bit_cast(uint16_t, (__bf16)lp_tanh(a));
     }
     __syncthreads();
+    // layer 3 on one tile: rows 0..3 are W3's (rows 4..15 repeat them and are
+    // not read), B the second-layer activations in two steps of 32; lane
+    // c < G of the first row of lanes then holds walker c's four outputs
+    const uint4 c0 = *reinterpret_cast<const uint4*>(hb + wv * kLpHid + 8 * q);
+    const uint4 c1 = *reinterpret_cast<const uint4*>(hb + wv * kLpHid + 32 + 8 * q);
+    const uint16_t* row3 = L.W + kLdsW3 + (c & (kLpOut - 1)) * kLpHid;
+    lp_f4 d = lp_tile(*reinterpret_cast<const uint4*>(row3 + 8 * q), c0, (lp_f4){0.f, 0.f, 0.f, 0.f});
+    d = lp_tile(*reinterpret_cast<const uint4*>(row3 + 32 + 8 * q), c1, d);
+    static_assert(kLpOut == 4, "the output rows are lanes c & 3");
 #pragma unroll
-    for (int r = 0; r < kLpOut; r++) o[r] = gsum<K>(tt[r]) + L.w(kLdsB3 + r);
+    for (int r = 0; r < kLpOut; r++) o[r] = __shfl(d[r], g, 64) + L.w(kLdsB3 + r);
 }
 
 // FP32 walks on matrix-core tiles (lp_mf): the features of electron i at
