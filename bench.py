@@ -464,11 +464,11 @@ def equilibrate(V, args):
         return eng.positions()
 
 
-def lp_engine(V, args, pos, walkers=None):
+def lp_engine(V, args, pos, walkers=None, refresh=None):
     W = walkers or args.walkers
     cfg = V.make_config(args.N, omega=args.omega, kinds=[V.SLATER, V.GAUSSIAN, V.PADE],
                         sampling=V.LEARNED, step_length=args.dt, quirks=0, precision=32,
-                        refresh_interval=args.refresh, num_walkers=W)
+                        refresh_interval=refresh or args.refresh, num_walkers=W)
     eng = V.Engine(cfg)
     eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
     eng.lp_set(lp_initial_weights(args.seed))
@@ -641,6 +641,18 @@ def main():
         t0 = time.perf_counter()
         hist = eng.lp_train(args.lp_train_iters, args.lp_train_steps, args.lp_lr)
         t_train = time.perf_counter() - t0
+        theta = eng.lp_get()
+        # the rebuild every --refresh accepted moves: every --refresh / a steps
+        # at the trained proposal's acceptance a (the Langevin forms accept
+        # nearly every move); the walk goes on from the trained engine's
+        # walkers and random stream
+        acc_train = float(hist[-5:, 1].mean())
+        lp_refresh = min(args.steps_per_iter, max(args.refresh, int(args.refresh / max(acc_train, 1e-3))))
+        post, (rs, rc) = eng.positions(), eng.random_state()
+        eng.close()
+        eng = lp_engine(V, args, post, refresh=lp_refresh)
+        eng.lp_set(theta)
+        eng.set_random_state(rs, rc)
         wl, kl, El, al = timed(eng, args.steps, args.warmup, args.steps_per_iter)
         rl = wsteps / wl
         learned = {"walker_steps_per_s": rl, "kernel_walker_steps_per_s": wsteps / kl,
@@ -650,12 +662,12 @@ def main():
                                 "learning_rate": args.lp_lr, "seconds": t_train,
                                 "esjd_start": float(hist[:5, 0].mean()),
                                 "esjd_end": float(hist[-5:, 0].mean()),
-                                "acceptance_end": float(hist[-5:, 1].mean())}}
-        theta = eng.lp_get()
+                                "acceptance_end": acc_train},
+                   "refresh_interval": lp_refresh}
         eng.close()
         if not args.no_iat:
             def make_lp(w):
-                e = lp_engine(V, args, pos, walkers=w)
+                e = lp_engine(V, args, pos, walkers=w, refresh=lp_refresh)
                 e.lp_set(theta)
                 return e
             with iat_engine(V, args, make_lp, pos, 23) as e:
@@ -727,11 +739,12 @@ def main():
                                 "to a node of the determinant are not frozen by its divergent "
                                 "drift" if args.drift_limit > 0 else
                                 "exact Metropolis-Hastings Langevin, no drift limit"),
-                   "refresh": (f"FP64 Slater-inverse rebuild every {args.refresh} steps of "
-                               "every walker (the FP32 Langevin form accepts 0.9999 of its "
-                               "moves, so that is 100 accepted moves; the learned form, which "
-                               f"accepts about 0.23, rebuilds every {args.refresh} steps too, "
-                               "about 23 accepted moves)")},
+                   "refresh": (f"FP64 Slater-inverse rebuild every {args.refresh} accepted "
+                               "moves of every walker on average: every "
+                               f"{args.refresh} steps in the FP32 Langevin form, which accepts "
+                               "0.9999 of its moves, and every refresh/a steps in the learned "
+                               "form, a = its acceptance after training (learned."
+                               "refresh_interval)")},
         "kernel_walker_steps_per_s": kern_rate,
         "energy": E32,
         "acceptance": acc32,
