@@ -21,6 +21,49 @@ def zero_variance_energy(N, omega):
     return E
 
 
+def closed_shell_mean_energy(N, omega, alpha):
+    """<E_L> under |psi|^2 for N non-interacting electrons of a 2-D oscillator
+    in closed shells, psi = det_up det_down exp(-alpha omega |r|^2 / 2) with
+    the closed-shell orbitals H_nx(sqrt(omega) x) H_ny(sqrt(omega) y), nx + ny
+    <= S (ref src/WaveFunctions/slaterdeterminant.cpp, gaussian.cpp,
+    src/Hamiltonians/harmonicoscillator.cpp).  For alpha != 1 the orbitals
+    phi_k = H_nx H_ny exp(-alpha omega r^2 / 2) are not orthogonal; a
+    determinant is unchanged by an invertible mixing of its orbitals, so the
+    mean energy is that of the orthogonalised ones, 2 Tr(S^-1 H) (both spins)
+    with S and H the orbitals' overlap and one-body Hamiltonian matrices
+    (H = -grad^2/2 + omega^2 r^2/2, the kinetic part as (1/2) int grad phi_k .
+    grad phi_l).  Every matrix element is a product of 1-D integrals of a
+    polynomial times exp(-alpha omega x^2), exact with Gauss-Hermite
+    quadrature.  (The per-shell Hermite polynomials of sqrt(omega) x and of
+    sqrt(alpha omega) x span the same space, so psi is the ground state of the
+    oscillator of frequency alpha omega and the result is also
+    E0 (alpha + 1/alpha) / 2 by the virial theorem; tests compare the two.)"""
+    from numpy.polynomial import hermite as Hm
+    NP, S = N // 2, 0
+    while (S + 1) * (S + 2) // 2 < NP:
+        S += 1
+    assert (S + 1) * (S + 2) // 2 == NP, N
+    orbs = [(nx, s - nx) for s in range(S + 1) for nx in range(s + 1)]
+    u, wq = Hm.hermgauss(40)
+    x = u / np.sqrt(alpha * omega)              # int f(x) e^{-alpha omega x^2} dx = sum wq f(x) / sqrt(alpha omega)
+    jac = 1.0 / np.sqrt(alpha * omega)
+    t = np.sqrt(omega) * x
+    Hv = [Hm.hermval(t, [0] * n + [1]) for n in range(S + 1)]
+    # d/dx [H_n(sqrt(omega) x) e^{-alpha omega x^2/2}] / e^{-alpha omega x^2/2}
+    dv = [np.sqrt(omega) * (2 * n * Hv[n - 1] if n else 0.0) - alpha * omega * x * Hv[n]
+          for n in range(S + 1)]
+    m = S + 1
+    s1 = np.array([[np.sum(wq * Hv[a] * Hv[b]) * jac for b in range(m)] for a in range(m)])
+    v1 = np.array([[np.sum(wq * x * x * Hv[a] * Hv[b]) * jac for b in range(m)] for a in range(m)])
+    t1 = np.array([[np.sum(wq * dv[a] * dv[b]) * jac for b in range(m)] for a in range(m)])
+    Sm = np.array([[s1[a[0], b[0]] * s1[a[1], b[1]] for b in orbs] for a in orbs])
+    Hmat = np.array([[0.5 * (t1[a[0], b[0]] * s1[a[1], b[1]] + s1[a[0], b[0]] * t1[a[1], b[1]]) +
+                      0.5 * omega ** 2 * (v1[a[0], b[0]] * s1[a[1], b[1]] +
+                                          s1[a[0], b[0]] * v1[a[1], b[1]])
+                      for b in orbs] for a in orbs])
+    return 2.0 * float(np.trace(np.linalg.solve(Sm, Hmat)))
+
+
 def oracle_config(N, omega, kinds, sampling, dt, interaction=True, quirks=O.Q2 | O.Q3,
                   drift_limit=0.0, basis_scaled=None):
     """The oracle with the reference's basis H_n(x); past N = 56 (shells of
