@@ -471,15 +471,19 @@ def lp_engine(V, args, pos, walkers=None, refresh=None):
                         refresh_interval=refresh or args.refresh, num_walkers=W)
     eng = V.Engine(cfg)
     eng.set_parameters(np.array([[1.0], [1.0], [1.0]]))
-    eng.lp_set(lp_initial_weights(args.seed))
+    eng.lp_set(lp_initial_weights(args.seed), *lp_hyper(args))
     eng.set_positions(pos[:W])
     eng.set_random_state(args.seed + 14, 0)
     return eng
 
 
+def lp_hyper(args):
+    return args.lp_sigma0, args.lp_mu_max, args.lp_ls_max
+
+
 def lp_initial_weights(seed):
     """Glorot-uniform hidden layers, zero head: the untrained proposal is a
-    symmetric Gaussian random walk of width sigma0 = 0.2/sqrt(omega)."""
+    symmetric Gaussian random walk of width sigma0/sqrt(omega) (--lp-sigma0)."""
     rng = np.random.default_rng(seed)
     th = np.zeros(V_LP_NPARAM, dtype=np.float32)
     a1, a2 = np.sqrt(6.0 / (18 + 64)), np.sqrt(6.0 / 128)
@@ -539,6 +543,13 @@ def main():
     ap.add_argument("--lp-train-iters", type=int, default=300)
     ap.add_argument("--lp-train-steps", type=int, default=10)
     ap.add_argument("--lp-lr", type=float, default=3e-3)
+    ap.add_argument("--lp-sigma0", type=float, default=0.4,
+                    help="learned proposal: move scale of the zero-head network, 1/sqrt(omega) "
+                         "units (the trained scale stays within sigma0 exp(+-ls_max))")
+    ap.add_argument("--lp-mu-max", type=float, default=1.0,
+                    help="learned proposal: bound of the mean move, 1/sqrt(omega) units")
+    ap.add_argument("--lp-ls-max", type=float, default=2.0,
+                    help="learned proposal: bound of |log(scale / sigma0)|")
     ap.add_argument("--fp64-iters", type=int, default=3)
     ap.add_argument("--cpu-steps-per-rank", type=int, default=200000)
     ap.add_argument("--cpu-port", action="store_true")
@@ -651,7 +662,7 @@ def main():
         post, (rs, rc) = eng.positions(), eng.random_state()
         eng.close()
         eng = lp_engine(V, args, post, refresh=lp_refresh)
-        eng.lp_set(theta)
+        eng.lp_set(theta, *lp_hyper(args))
         eng.set_random_state(rs, rc)
         wl, kl, El, al = timed(eng, args.steps, args.warmup, args.steps_per_iter)
         rl = wsteps / wl
@@ -663,12 +674,14 @@ def main():
                                 "esjd_start": float(hist[:5, 0].mean()),
                                 "esjd_end": float(hist[-5:, 0].mean()),
                                 "acceptance_end": acc_train},
+                   "hyper": {"sigma0": args.lp_sigma0, "mu_max": args.lp_mu_max,
+                             "ls_max": args.lp_ls_max},
                    "refresh_interval": lp_refresh}
         eng.close()
         if not args.no_iat:
             def make_lp(w):
                 e = lp_engine(V, args, pos, walkers=w, refresh=lp_refresh)
-                e.lp_set(theta)
+                e.lp_set(theta, *lp_hyper(args))
                 return e
             with iat_engine(V, args, make_lp, pos, 23) as e:
                 itl = measure_iat(e, args.lp_iat_steps, args.lp_iat_block)
